@@ -1,0 +1,216 @@
+/*
+ * jmm_engine.h -- C ABI of the MI355X (gfx950) Metropolis engine for 1D
+ * classical particles: many independent chains of single-particle
+ * displacement trials plus box-length trials, run by hand-written HIP kernels.
+ *
+ * Every entry point returns 0 on success or a negative JMM_E* code; the text
+ * of the last error on the calling thread is available from jmm_last_error().
+ * No entry point throws or exits.  An engine handle is not thread-safe: call
+ * it from one host thread.  All device work is queued on the engine's own HIP
+ * stream; host output arrays are written only by the calls that document it.
+ *
+ * Citations are to the reference repository (mmansell7/jmmOneDMC); each
+ * entry point names the reference interface it replaces.
+ */
+#ifndef JMM_ENGINE_H
+#define JMM_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JMM_ABI_VERSION 1
+
+/* status codes */
+#define JMM_OK 0
+#define JMM_EINVAL (-1)   /* bad argument or chain specification */
+#define JMM_EHIP (-2)     /* HIP runtime failure */
+#define JMM_ENOMEM (-3)   /* device or host allocation failed */
+#define JMM_EFULL (-4)    /* sample buffer full: read samples first */
+#define JMM_ESTATE (-5)   /* call not valid in the engine's current state */
+
+/* potentials (reference POT keyword, src/readInput.cpp:104-131, plus the
+ * configurations' hard-rod/square-well and Yukawa potentials) */
+#define JMM_POT_LJ 0       /* 4 eps((s/d)^12-(s/d)^6); p = {eps, sigma}; inclusive cutoff;
+                              cutoff = INFINITY is the reference's "LJ" */
+#define JMM_POT_HARMONIC 1 /* (d-1)^2, 1e11 for d <= 0, strict cutoff (reference HARMONIC) */
+#define JMM_POT_HRSW 2     /* hard rod p0, well depth p1 out to p2 (strict) */
+#define JMM_POT_YUKAWA 3   /* p0 exp(-p1 d)/d, hard core p2, inclusive cutoff */
+
+#define JMM_WALLS 0        /* hard walls at +-L/2, reference behaviour */
+#define JMM_PERIODIC 1     /* periodic line of length L */
+#define JMM_NLT 0          /* constant length (reference "NLT", i.e. NVT) */
+#define JMM_NPT 1          /* constant pressure: one box-length trial per sweep */
+#define JMM_VOL_LINEAR 0   /* L' = L + U(-dv, dv), Jacobian N ln s (reference fav) */
+#define JMM_VOL_LOG 1      /* ln L' = ln L + U(-dv, dv), Jacobian (N+1) ln s */
+
+/* execution forms (all three run the same Markov chain) */
+#define JMM_FORM_PLAIN_FP64 0   /* one chain per wavefront, trials one at a time,
+                                   energy change = wavefront reduction over neighbours */
+#define JMM_FORM_BATCHED_FP64 1 /* persistent work-queue kernel, one lane per block
+                                   of particles, chains packed into workgroups */
+#define JMM_FORM_BATCHED_FP32 2 /* as BATCHED_FP64 with FP32 pair arithmetic,
+                                   compensated energy change, FP64 accumulators */
+
+/* One chain: the hot-path fields of the reference's MCInput
+ * (src/jmmMCState.h:66-101) plus the engine's additive fields. */
+typedef struct jmm_chain_spec {
+    int32_t potential;  /* JMM_POT_* (reference POT) */
+    int32_t boundary;   /* JMM_WALLS / JMM_PERIODIC */
+    int32_t ensemble;   /* JMM_NLT / JMM_NPT (reference ENSEMBLE) */
+    int32_t vol_move;   /* JMM_VOL_LINEAR / JMM_VOL_LOG */
+    int32_t nbn;        /* reference NBN: partners farther than nbn in index
+                           order are ignored; -1 = no limit */
+    int32_t shifted;    /* 1: phi(d) - phi(cutoff) inside the cutoff */
+    uint64_t N;         /* reference N */
+    double T;           /* reference T (k_B = 1) */
+    double P;           /* reference P */
+    double L0;          /* initial length; lattice start x_i = ((i+1/2)/N - 1/2) L0 */
+    double max_step;    /* reference MAXSTEP: displacement uniform in +-max_step */
+    double max_dv;      /* reference MAXDV (linear) or the ln L half-width (log) */
+    double cutoff;      /* reference POT cutoff; INFINITY allowed with walls */
+    double pot_p[4];    /* potential parameters, see JMM_POT_* */
+    uint64_t seed;      /* reference SEED: Philox4x32-10 key */
+    uint32_t chain_id;  /* Philox counter word: distinct streams per seed */
+    uint32_t eci;       /* reference ENGCHECK, in sweeps: FP64 recompute interval, 0 = never */
+    uint32_t bcap;      /* largest number of particle blocks (power of two, 0 = 256) */
+    uint32_t reserved;
+} jmm_chain_spec;
+
+typedef struct jmm_engine_opts {
+    int32_t form;             /* JMM_FORM_* */
+    int32_t device;           /* HIP device ordinal */
+    uint32_t sample_capacity; /* per-chain sweeps held on device between reads (0 = 1024) */
+    uint32_t block_sweeps;    /* sweeps per kernel launch (0 = 100) */
+    uint32_t wg_threads;      /* batched forms: lanes per workgroup, 64..1024 (0 = 256) */
+    uint32_t use_graph;       /* 1: replay sweep blocks from a captured hipGraph */
+    uint32_t log_accepts;     /* 1: record one byte per displacement trial (tests) */
+    uint32_t lds_budget;      /* batched forms: LDS bytes per workgroup (0 = 64 KiB) */
+    uint32_t reserved[4];
+} jmm_engine_opts;
+
+/* One per chain per sweep (the per-sweep form of the reference's per-trial
+ * updateThermo, src/jmmMCState.cpp:1941-1961). */
+typedef struct jmm_sample {
+    double U;       /* configurational energy after the sweep */
+    double L;       /* box length after the sweep */
+    uint32_t acc_d; /* accepted displacement trials in the sweep */
+    uint32_t acc_v; /* accepted box-length trials in the sweep (0/1) */
+} jmm_sample;
+
+/* Cumulative per-chain counters (reference dAcc/vAcc, src/jmmMCState.cpp:66-69). */
+typedef struct jmm_counters {
+    uint64_t sweeps;
+    uint64_t trials;        /* displacement trials */
+    uint64_t acc_d;         /* accepted displacements */
+    uint64_t wall_rejects;  /* rejected at the walls */
+    uint64_t core_rejects;  /* rejected by the hard-core rule (crossings included) */
+    uint64_t pair_terms;    /* pair evaluations in displacement trials (n_old + n_new) */
+    uint64_t vol_trials;
+    uint64_t acc_v;
+    uint64_t recomputes;    /* FP64 full-energy recomputes */
+    double max_drift;       /* largest |U_running - U_recomputed| seen */
+    double U;               /* current running energy */
+    double L;               /* current box length */
+} jmm_counters;
+
+typedef struct jmm_engine jmm_engine;
+
+/* Library identity; no reference counterpart. */
+int jmm_abi_version(void);
+const char *jmm_last_error(void);
+
+/* The trial schedule of one chain: B blocks of particles, M_min = smallest
+ * index separation of trials that run at once, w = largest index distance of
+ * an interacting partner.  Pure host function; no reference counterpart (the
+ * reference draws trial types with replacement, src/jmmMCState.cpp:1762). */
+int jmm_schedule(const jmm_chain_spec *spec, uint64_t *B, uint64_t *M_min, uint64_t *w);
+
+/* Replaces setupMCS (src/jmmMCState.cpp:261-790: parameter copy :267-290,
+ * potential select :291-399, ensemble :401-425, lattice :549-562, RNG
+ * :779-781) and the first-step fad energy (src/Main.cpp:66-68), for n_chains
+ * chains at once.  Uploads the lattice start and computes each chain's FP64
+ * energy.  *out receives the handle. */
+int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts *opts,
+               jmm_engine **out);
+
+/* Replaces freeMCS (src/jmmMCState.cpp:792-842). */
+int jmm_destroy(jmm_engine *e);
+
+/* Replaces the chain loop of src/Main.cpp:114-180 (incrementStep
+ * src/jmmMCState.cpp:1734-1754 + Step :1758-1811): n_sweeps sweeps on every
+ * chain, each sweep = N displacement trials (qad2, :1160-1464) in the
+ * engine's block order, then one box-length trial for NPT chains (fav,
+ * :2161-2293), then the ECheck-style FP64 recompute (:1965-2095) when due.
+ * Asynchronous; samples accumulate on the device. */
+int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps);
+
+/* Waits for queued work; no reference counterpart. */
+int jmm_synchronize(jmm_engine *e);
+
+/* Replaces ECheck's recompute (src/jmmMCState.cpp:1973-1993) as a query:
+ * U_out[c] = FP64 energy of chain c's current configuration.  Synchronous;
+ * does not modify the chains. */
+int jmm_full_energy(jmm_engine *e, double *U_out);
+
+/* Replaces updateThermo/printThermo's data path (src/jmmMCState.cpp:
+ * 1896-1961): copies every chain's samples since the last read into
+ * out[c * max_sweeps + s] and empties the device buffer.  *n_out = sweeps
+ * copied per chain (all chains have the same count).  Synchronous. */
+int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out);
+
+/* Replaces printCoords (src/jmmMCState.cpp:1007-1017) and the restart read
+ * (:572-765): chain c's positions (sorted), length, running energy and sweep
+ * counter.  Any output pointer may be NULL.  Synchronous. */
+int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uint64_t *sweep);
+
+/* Restart: installs positions (must be sorted and respect the hard core),
+ * length and sweep counter; the running energy is recomputed in FP64.
+ * Synchronous. */
+int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t sweep);
+
+/* Fault injection for the drift guard: adds delta to chain c's running
+ * energy (the next recompute must detect and repair it).  Synchronous. */
+int jmm_perturb_energy(jmm_engine *e, int chain, double delta);
+
+/* Replaces printAcc/printE (src/jmmMCState.cpp:2143-2156). Synchronous. */
+int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out);
+
+/* Accept log (opts.log_accepts): out[c][s][i] one byte per displacement
+ * trial for the sweeps since the last read, c in chain order, particle index
+ * order; same read/drain rules as jmm_read_samples.  out must hold
+ * sum_c N_c * max_sweeps bytes; chain c starts at offset
+ * max_sweeps * sum_{c'<c} N_c'. */
+int jmm_read_accept_log(jmm_engine *e, uint8_t *out, uint32_t max_sweeps, uint32_t *n_out);
+
+/* Kernel timing of the sweep kernel, from HIP events recorded on the
+ * engine's stream around every sweep-block launch since the last reset:
+ * number of launches, total milliseconds, total sweeps.  Synchronous. */
+int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *sweeps, int reset);
+
+/* Observables of every chain's current configuration, for the
+ * differentiation path (no reference counterpart; the reference keeps the
+ * virial totals Vir/Vir6/Vir12, src/jmmMCState.cpp:86-96):
+ * obs[c*6 + 0..5] = U, W = -sum d phi'(d), dU/dtheta0, dU/dtheta1,
+ * dW/dtheta0, dW/dtheta1 (theta = (eps, sigma) for LJ, (A, kappa) for
+ * Yukawa).  grad (nullable) receives dU/dx for all chains, chain c at offset
+ * sum_{c'<c} N_c'.  FP32 pair arithmetic for the FP32 form, FP64 otherwise;
+ * reductions in FP64.  Synchronous. */
+int jmm_observables(jmm_engine *e, double *obs, double *grad);
+
+/* The backward kernel on caller-owned DEVICE memory (for the autograd
+ * binding): n_configs configurations of N particles of one potential.
+ * x_dev[n_configs*N] (sorted per configuration), L_dev[n_configs];
+ * obs_dev[n_configs*6] as in jmm_observables; grad_dev (nullable)
+ * [n_configs*N] = dU/dx.  precision: 32 or 64.  stream: hipStream_t or NULL.
+ * Asynchronous on that stream.  No reference counterpart. */
+int jmm_pair_observables(const jmm_chain_spec *spec, int n_configs, const double *x_dev,
+                         const double *L_dev, double *obs_dev, double *grad_dev, int precision,
+                         void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,867 @@
+// engine.hip -- host side of the engine: chain validation, the trial
+// schedule, bundle packing, device buffers, sweep-block launches (optionally
+// replayed from a captured hipGraph), sample read-back and the C ABI declared
+// in include/jmm_engine.h.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "jmm_engine.h"
+#include "kernels.h"
+
+using namespace jmm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(JMM_EHIP, "%s: %s", #x, hipGetErrorString(e_));  \
+    } while (0)
+
+double gmin_of(const jmm_chain_spec &s)
+{
+    switch (s.potential) {
+    case JMM_POT_LJ: return 0.5 * s.pot_p[1];
+    case JMM_POT_HRSW: return s.pot_p[0];
+    case JMM_POT_YUKAWA: return s.pot_p[2];
+    default: return std::numeric_limits<double>::denorm_min();  // harmonic: gap <= 0
+    }
+}
+
+// The trial schedule (also restated by the oracle): w = largest index distance
+// of a partner that can interact with a trial (hard core g_min keeps every gap
+// >= g_min, so a partner within rc of the old or new position is at most
+// (rc + delta)/g_min places away); M_min = smallest index separation of two
+// trials that can run at once (their partner sets and moved particles must
+// stay apart: (rc + 2 delta)/g_min, or NBN + 1); B = power-of-two number of
+// blocks with every block at least M_min long, capped by bcap.
+void schedule_of(const jmm_chain_spec &s, uint64_t &B, uint64_t &M, uint64_t &w)
+{
+    const uint64_t N = s.N;
+    const uint64_t big = N > 0 ? N : 1;
+    uint64_t wc = big, mc = big + 1;
+    const double g = gmin_of(s);
+    if (std::isfinite(s.cutoff) && g > 1e-300) {
+        const double a = std::floor((s.cutoff + s.max_step) / g);
+        const double b = std::floor((s.cutoff + 2.0 * s.max_step) / g) + 1.0;
+        if (a < (double)big) wc = (uint64_t)a;
+        if (b < (double)(big + 1)) mc = (uint64_t)b;
+    }
+    if (s.nbn >= 0) {
+        if ((uint64_t)s.nbn < wc) wc = (uint64_t)s.nbn;
+        if ((uint64_t)s.nbn + 1 < mc) mc = (uint64_t)s.nbn + 1;
+    }
+    if (N >= 1 && wc > N - 1) wc = N - 1;
+    uint64_t nb = 1;
+    if (mc >= 1 && mc <= N) {
+        uint64_t q = N / mc;
+        const uint64_t cap = s.bcap ? s.bcap : 256;
+        if (q > cap) q = cap;
+        while (nb * 2 <= q) nb *= 2;
+    }
+    B = nb;
+    M = mc;
+    w = wc;
+}
+
+int validate(const jmm_chain_spec &s, int idx)
+{
+    if (s.N < 1 || s.N > (1ull << 30)) return fail(JMM_EINVAL, "chain %d: N=%llu out of range", idx, (unsigned long long)s.N);
+    if (!(s.T > 0.0)) return fail(JMM_EINVAL, "chain %d: T must be > 0", idx);
+    if (!(s.max_step >= 0.0) || !std::isfinite(s.max_step)) return fail(JMM_EINVAL, "chain %d: bad max_step", idx);
+    if (s.potential < 0 || s.potential > 3) return fail(JMM_EINVAL, "chain %d: unknown potential %d", idx, s.potential);
+    if (s.boundary != JMM_WALLS && s.boundary != JMM_PERIODIC) return fail(JMM_EINVAL, "chain %d: bad boundary", idx);
+    if (s.ensemble != JMM_NLT && s.ensemble != JMM_NPT) return fail(JMM_EINVAL, "chain %d: bad ensemble", idx);
+    if (s.vol_move != JMM_VOL_LINEAR && s.vol_move != JMM_VOL_LOG) return fail(JMM_EINVAL, "chain %d: bad vol_move", idx);
+    if (!(s.cutoff > 0.0)) return fail(JMM_EINVAL, "chain %d: cutoff must be > 0", idx);
+    if (!(s.L0 > 0.0) || !std::isfinite(s.L0)) return fail(JMM_EINVAL, "chain %d: L0 must be finite and > 0", idx);
+    if (s.potential == JMM_POT_LJ && !(s.pot_p[1] > 0.0)) return fail(JMM_EINVAL, "chain %d: LJ sigma must be > 0", idx);
+    if (s.potential == JMM_POT_HRSW) {
+        if (!(s.pot_p[0] > 0.0) || !(s.pot_p[2] >= s.pot_p[0]))
+            return fail(JMM_EINVAL, "chain %d: hard-rod/square-well needs 0 < rod <= well edge", idx);
+        if (s.cutoff != s.pot_p[2]) return fail(JMM_EINVAL, "chain %d: hard-rod/square-well cutoff must equal the well edge", idx);
+    }
+    if (s.potential == JMM_POT_YUKAWA && !(s.pot_p[2] > 0.0)) return fail(JMM_EINVAL, "chain %d: Yukawa hard core must be > 0", idx);
+    if (s.boundary == JMM_PERIODIC) {
+        if (!std::isfinite(s.cutoff)) return fail(JMM_EINVAL, "chain %d: periodic chains need a finite cutoff", idx);
+        if (!(s.L0 > 2.0 * (s.cutoff + s.max_step)))
+            return fail(JMM_EINVAL, "chain %d: periodic length must exceed 2 (cutoff + max_step)", idx);
+    }
+    if (s.N >= 2 && (s.L0 / (double)s.N) < gmin_of(s))
+        return fail(JMM_EINVAL, "chain %d: lattice spacing L0/N is inside the hard core", idx);
+    if (s.ensemble == JMM_NPT && !(s.max_dv >= 0.0)) return fail(JMM_EINVAL, "chain %d: bad max_dv", idx);
+    return 0;
+}
+
+Pot make_pot(const jmm_chain_spec &s)
+{
+    Pot q{};
+    q.rc = s.cutoff;
+    q.p0 = s.pot_p[0];
+    q.p1 = s.pot_p[1];
+    q.p2 = s.pot_p[2];
+    const double sig3 = s.pot_p[1] * s.pot_p[1] * s.pot_p[1];
+    q.sig6 = sig3 * sig3;
+    q.e4 = 4.0 * s.pot_p[0];
+    q.shift = 0.0;
+    if (s.shifted && std::isfinite(s.cutoff)) {
+        switch (s.potential) {
+        case JMM_POT_LJ: q.shift = phi64<POT_LJ>(q, s.cutoff); break;
+        case JMM_POT_YUKAWA: q.shift = phi64<POT_YUKAWA>(q, s.cutoff); break;
+        default: break;  // harmonic / square well: no shifted form
+        }
+    }
+    q.rcf = (float)s.cutoff;
+    q.shiftf = (float)q.shift;
+    q.e4f = (float)q.e4;
+    q.sig6f = (float)q.sig6;
+    q.p0f = (float)q.p0;
+    q.p1f = (float)q.p1;
+    q.p2f = (float)q.p2;
+    return q;
+}
+
+// Host FP64 energy in the oracle's order (every pair once, from its left
+// member, nearest partner first): the initial energy (fad at step 0,
+// src/Main.cpp:66-68) and the restart energy.
+template <int KIND>
+double host_energy_t(const Pot &q, const double *x, int64_t N, bool per, double L, uint64_t w)
+{
+    double U = 0.0;
+    for (int64_t i = 0; i < N; ++i)
+        for (uint64_t k = 1; k <= w; ++k) {
+            const int64_t j = i + (int64_t)k;
+            double xj;
+            if (j < N) xj = x[j];
+            else if (per) xj = x[j - N] + L;
+            else break;
+            const double d = xj - x[i];
+            if (d > q.rc) break;
+            U += phi64<KIND>(q, d);
+        }
+    return U;
+}
+
+double host_energy(const jmm_chain_spec &s, const Pot &q, const double *x, double L, uint64_t w)
+{
+    const bool per = s.boundary == JMM_PERIODIC;
+    switch (s.potential) {
+    case JMM_POT_LJ: return host_energy_t<POT_LJ>(q, x, (int64_t)s.N, per, L, w);
+    case JMM_POT_HARMONIC: return host_energy_t<POT_HARMONIC>(q, x, (int64_t)s.N, per, L, w);
+    case JMM_POT_HRSW: return host_energy_t<POT_HRSW>(q, x, (int64_t)s.N, per, L, w);
+    default: return host_energy_t<POT_YUKAWA>(q, x, (int64_t)s.N, per, L, w);
+    }
+}
+
+// Shifted-form values at the cutoff for the observables (E, dE/dtheta0,
+// dE/dtheta1), computed with the unshifted pair function.
+void shift_obs(const jmm_chain_spec &s, const Pot &q, double sh[3])
+{
+    sh[0] = sh[1] = sh[2] = 0.0;
+    if (!s.shifted || !std::isfinite(s.cutoff)) return;
+    Pot q0 = q;
+    q0.shift = 0.0;
+    const double z[3] = {0, 0, 0};
+    double o[6];
+    if (s.potential == JMM_POT_LJ) pair_obs<POT_LJ, double>(q0, s.cutoff, z, o);
+    else if (s.potential == JMM_POT_YUKAWA) pair_obs<POT_YUKAWA, double>(q0, s.cutoff, z, o);
+    else return;
+    sh[0] = o[0];
+    sh[1] = o[2];
+    sh[2] = o[3];
+}
+
+DevChain make_chain(const jmm_chain_spec &s)
+{
+    DevChain c{};
+    uint64_t B, M, w;
+    schedule_of(s, B, M, w);
+    c.seed = s.seed;
+    c.N = (uint32_t)s.N;
+    c.B = (uint32_t)B;
+    c.w = (uint32_t)w;
+    c.smax = (uint32_t)((s.N + B - 1) / B);
+    c.chain_id = s.chain_id;
+    c.eci = s.eci;
+    c.flags = (s.boundary == JMM_PERIODIC ? F_PERIODIC : 0u) | (s.ensemble == JMM_NPT ? F_NPT : 0u) |
+              (s.vol_move == JMM_VOL_LOG ? F_VOLLOG : 0u);
+    c.kind = (uint32_t)s.potential;
+    c.beta = 1.0 / s.T;
+    c.betaf = (float)c.beta;
+    c.P = s.P;
+    c.max_step = s.max_step;
+    c.max_dv = s.max_dv;
+    c.gmin = gmin_of(s);
+    c.Lmin = s.boundary == JMM_PERIODIC ? 2.0 * (s.cutoff + s.max_step) : 0.0;
+    c.pot = make_pot(s);
+    shift_obs(s, c.pot, c.sh);
+    return c;
+}
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(JMM_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+struct Group {
+    int gx = 0;
+    uint32_t n_items = 0;     // bundles (batched) or chains (plain)
+    uint32_t grid = 0;
+    size_t lds = 0;
+    DevBundle *d_bundles = nullptr;
+    uint32_t *d_list = nullptr;
+};
+
+struct jmm_engine {
+    int n = 0, form = 0, device = 0, T = 256;
+    hipStream_t stream = nullptr;
+    std::vector<jmm_chain_spec> specs;
+    std::vector<DevChain> hch;          // internal order
+    std::vector<uint32_t> int_of_ext, ext_of_int;
+    std::vector<uint64_t> ext_off;      // external chain -> offset in an ext-ordered flat array
+    uint64_t total_N = 0;
+    DevChain *d_chains = nullptr;
+    DevState *d_states = nullptr;
+    double *d_x = nullptr;
+    DevSlot *d_slots = nullptr;
+    Ctrl *d_ctrl = nullptr;
+    double *d_sU = nullptr, *d_sL = nullptr;
+    uint2 *d_sA = nullptr;
+    uint8_t *d_log = nullptr;
+    double *d_obs = nullptr, *d_grad = nullptr, *d_Ltmp = nullptr;
+    uint32_t *d_all = nullptr;
+    uint32_t cap = 1024, write_pos = 0, block_sweeps = 100;
+    Group grp[2];
+    int use_graph = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    uint64_t ev_sweeps = 0;
+};
+
+namespace {
+
+int launch_block(jmm_engine *e, uint32_t m)
+{
+    for (int g = 0; g < 2; ++g) {
+        Group &G = e->grp[g];
+        if (!G.n_items) continue;
+        HIPCHK(launch_begin(&e->d_ctrl[g], m, e->stream));
+        SweepArgs a{};
+        a.chains = e->d_chains;
+        a.states = e->d_states;
+        a.x = e->d_x;
+        a.bundles = G.d_bundles;
+        a.slots = e->d_slots;
+        a.plain_list = G.d_list;
+        a.ctrl = &e->d_ctrl[g];
+        a.sU = e->d_sU;
+        a.sL = e->d_sL;
+        a.sA = e->d_sA;
+        a.acclog = e->d_log;
+        a.n_items = G.n_items;
+        a.n_sweeps = m;
+        a.n_chains = (uint32_t)e->n;
+        a.cap = e->cap;
+        if (e->form == JMM_FORM_PLAIN_FP64) {
+            HIPCHK(launch_plain(a, G.gx, G.grid, G.lds, e->stream));
+        } else {
+            HIPCHK(launch_batched(a, e->form == JMM_FORM_BATCHED_FP32, e->T, G.gx, G.grid, G.lds,
+                                  e->stream));
+        }
+    }
+    return 0;
+}
+
+int set_lds_attr(const void *fn, size_t lds)
+{
+    if (lds > 65536) {
+        hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (err != hipSuccess) return fail(JMM_EHIP, "hipFuncSetAttribute(%zu bytes LDS): %s", lds, hipGetErrorString(err));
+    }
+    return 0;
+}
+
+// Pack chains into bundles: chains of equal (potential, ensemble, boundary,
+// recompute interval) share a workgroup; block counts are powers of two and
+// are placed largest first, so every chain's lane range is aligned to its own
+// size (the segmented reductions rely on this).
+struct PackOut {
+    std::vector<DevBundle> b[2];
+    std::vector<DevSlot> slots;
+    std::vector<uint32_t> order;  // internal -> external
+};
+
+PackOut pack(const std::vector<jmm_chain_spec> &specs, const std::vector<DevChain> &ch, int T,
+             size_t lds_doubles)
+{
+    const int n = (int)specs.size();
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    auto key = [&](uint32_t c) {
+        const jmm_chain_spec &s = specs[c];
+        return std::make_tuple(s.potential, s.ensemble, s.boundary, s.eci);
+    };
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        if (key(a) != key(b)) return key(a) < key(b);
+        if (ch[a].B != ch[b].B) return ch[a].B > ch[b].B;
+        if (ch[a].smax != ch[b].smax) return ch[a].smax > ch[b].smax;
+        return ch[a].N > ch[b].N;
+    });
+    PackOut o;
+    // global-x chains first (they run in their own group), then LDS bundles
+    std::vector<uint32_t> gx, lds;
+    for (uint32_t c : idx) ((size_t)ch[c].N > lds_doubles ? gx : lds).push_back(c);
+    uint64_t xoff = 0;
+    auto open_bundle = [&](int g, uint32_t c) {
+        DevBundle bd{};
+        bd.slot0 = (uint32_t)o.slots.size();
+        bd.kind = ch[c].kind;
+        bd.global_x = g;
+        bd.x_off = xoff;
+        o.b[g].push_back(bd);
+    };
+    for (uint32_t c : gx) {
+        open_bundle(1, c);
+        DevBundle &bd = o.b[1].back();
+        o.slots.push_back({(uint32_t)o.order.size(), 0u, ch[c].B, 0u});
+        bd.nslots = 1;
+        bd.lanes = ch[c].B;
+        bd.smax = ch[c].smax;
+        bd.lds_doubles = 0;
+        o.order.push_back(c);
+        xoff += ch[c].N;
+    }
+    for (size_t k = 0; k < lds.size(); ++k) {
+        const uint32_t c = lds[k];
+        bool fresh = o.b[0].empty();
+        if (!fresh) {
+            const DevBundle &bd = o.b[0].back();
+            const uint32_t c0 = o.order[o.slots[bd.slot0].chain];
+            fresh = key(c0) != key(c) || bd.lanes + ch[c].B > (uint32_t)T ||
+                    (size_t)bd.lds_doubles + ch[c].N > lds_doubles;
+        }
+        if (fresh) open_bundle(0, c);
+        DevBundle &bd = o.b[0].back();
+        o.slots.push_back({(uint32_t)o.order.size(), bd.lanes, ch[c].B, bd.lds_doubles});
+        bd.nslots++;
+        bd.lanes += ch[c].B;
+        bd.smax = std::max(bd.smax, ch[c].smax);
+        bd.lds_doubles += ch[c].N;
+        o.order.push_back(c);
+        xoff += ch[c].N;
+    }
+    return o;
+}
+
+double bundle_cost(const DevBundle &b, const std::vector<DevChain> &chi, const std::vector<DevSlot> &slots)
+{
+    const DevChain &c = chi[slots[b.slot0].chain];
+    return (double)b.smax * (double)(c.w + 2) + ((c.flags & F_NPT) ? (double)c.smax * c.w : 0.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int jmm_abi_version(void) { return JMM_ABI_VERSION; }
+
+const char *jmm_last_error(void) { return g_err.c_str(); }
+
+int jmm_schedule(const jmm_chain_spec *spec, uint64_t *B, uint64_t *M_min, uint64_t *w)
+{
+    if (!spec || !B || !M_min || !w) return fail(JMM_EINVAL, "jmm_schedule: null argument");
+    schedule_of(*spec, *B, *M_min, *w);
+    return 0;
+}
+
+int jmm_destroy(jmm_engine *e)
+{
+    if (!e) return 0;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (auto &p : e->ev) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    (void)hipFree(e->d_chains);
+    (void)hipFree(e->d_states);
+    (void)hipFree(e->d_x);
+    (void)hipFree(e->d_slots);
+    (void)hipFree(e->d_ctrl);
+    (void)hipFree(e->d_sU);
+    (void)hipFree(e->d_sL);
+    (void)hipFree(e->d_sA);
+    (void)hipFree(e->d_log);
+    (void)hipFree(e->d_obs);
+    (void)hipFree(e->d_grad);
+    (void)hipFree(e->d_Ltmp);
+    (void)hipFree(e->d_all);
+    for (auto &G : e->grp) {
+        (void)hipFree(G.d_bundles);
+        (void)hipFree(G.d_list);
+    }
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts *opts,
+               jmm_engine **out)
+{
+    if (!out) return fail(JMM_EINVAL, "jmm_create: out is NULL");
+    *out = nullptr;
+    if (!specs || n_chains <= 0) return fail(JMM_EINVAL, "jmm_create: need at least one chain");
+    jmm_engine_opts o{};
+    if (opts) o = *opts;
+    if (o.form < 0 || o.form > 2) return fail(JMM_EINVAL, "jmm_create: unknown form %d", o.form);
+    for (int c = 0; c < n_chains; ++c)
+        if (int rc = validate(specs[c], c)) return rc;
+
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (o.device < 0 || o.device >= ndev) return fail(JMM_EINVAL, "jmm_create: device %d of %d", o.device, ndev);
+    HIPCHK(hipSetDevice(o.device));
+
+    jmm_engine *e = new jmm_engine();
+    e->n = n_chains;
+    e->form = o.form;
+    e->device = o.device;
+    e->cap = o.sample_capacity ? o.sample_capacity : 1024;
+    e->block_sweeps = o.block_sweeps ? o.block_sweeps : 100;
+    e->use_graph = o.use_graph ? 1 : 0;
+    e->specs.assign(specs, specs + n_chains);
+    auto bail = [&](int rc) {
+        jmm_destroy(e);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(JMM_EHIP, "hipStreamCreate failed"));
+
+    std::vector<DevChain> ch(n_chains);
+    uint32_t maxB = 1;
+    for (int c = 0; c < n_chains; ++c) {
+        ch[c] = make_chain(specs[c]);
+        maxB = std::max(maxB, ch[c].B);
+    }
+    int lds_max = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device));
+    int n_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device));
+
+    // internal chain order and bundles
+    std::vector<DevBundle> bundles[2];
+    std::vector<DevSlot> slots;
+    std::vector<uint32_t> order(n_chains);
+    if (e->form == JMM_FORM_PLAIN_FP64) {
+        std::iota(order.begin(), order.end(), 0u);
+    } else {
+        e->T = (int)std::max<uint32_t>(o.wg_threads ? o.wg_threads : 256u, maxB) <= 256 ? 256 : 1024;
+        if (maxB > 1024) return bail(fail(JMM_EINVAL, "jmm_create: more than 1024 blocks in a chain"));
+        size_t budget = o.lds_budget ? o.lds_budget : 65536;
+        budget = std::min<size_t>(budget, (size_t)lds_max);
+        PackOut p = pack(e->specs, ch, e->T, budget / 8);
+        order = p.order;
+        slots = p.slots;
+        for (int g = 0; g < 2; ++g) bundles[g] = p.b[g];
+    }
+    e->ext_of_int = order;
+    e->int_of_ext.assign(n_chains, 0);
+    for (int k = 0; k < n_chains; ++k) e->int_of_ext[order[k]] = (uint32_t)k;
+    e->ext_off.assign(n_chains, 0);
+    uint64_t acc = 0;
+    for (int c = 0; c < n_chains; ++c) {
+        e->ext_off[c] = acc;
+        acc += specs[c].N;
+    }
+    e->total_N = acc;
+
+    // internal-order chain constants, positions, states
+    e->hch.resize(n_chains);
+    std::vector<double> hx(e->total_N);
+    std::vector<DevState> hst(n_chains);
+    uint64_t xoff = 0;
+    for (int k = 0; k < n_chains; ++k) {
+        const uint32_t c = order[k];
+        DevChain d = ch[c];
+        d.x_off = xoff;
+        d.log_off = xoff * e->cap;
+        const jmm_chain_spec &s = specs[c];
+        double *x = hx.data() + xoff;
+        for (uint64_t i = 0; i < s.N; ++i) x[i] = (((double)i + 0.5) / (double)s.N - 0.5) * s.L0;
+        DevState st{};
+        st.L = s.L0;
+        st.U = host_energy(s, d.pot, x, s.L0, d.w);
+        hst[k] = st;
+        e->hch[k] = d;
+        xoff += s.N;
+    }
+    if (int rc = dalloc(&e->d_chains, n_chains)) return bail(rc);
+    if (int rc = dalloc(&e->d_states, n_chains)) return bail(rc);
+    if (int rc = dalloc(&e->d_x, e->total_N)) return bail(rc);
+    if (int rc = dalloc(&e->d_ctrl, 2)) return bail(rc);
+    if (int rc = dalloc(&e->d_sU, (size_t)e->cap * n_chains)) return bail(rc);
+    if (int rc = dalloc(&e->d_sL, (size_t)e->cap * n_chains)) return bail(rc);
+    if (int rc = dalloc(&e->d_sA, (size_t)e->cap * n_chains)) return bail(rc);
+    if (o.log_accepts)
+        if (int rc = dalloc(&e->d_log, (size_t)e->cap * e->total_N)) return bail(rc);
+    HIPCHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_states, hst.data(), sizeof(DevState) * n_chains, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
+    Ctrl zc[2] = {};
+    HIPCHK(hipMemcpy(e->d_ctrl, zc, sizeof zc, hipMemcpyHostToDevice));
+
+    // launch groups
+    if (e->form == JMM_FORM_PLAIN_FP64) {
+        std::vector<uint32_t> lists[2];
+        size_t maxlds[2] = {0, 0};
+        for (int k = 0; k < n_chains; ++k) {
+            const size_t need = sizeof(double) * e->hch[k].N;
+            const int g = need > (size_t)lds_max ? 1 : 0;
+            lists[g].push_back((uint32_t)k);
+            if (!g) maxlds[0] = std::max(maxlds[0], need);
+        }
+        for (int g = 0; g < 2; ++g) {
+            Group &G = e->grp[g];
+            G.gx = g;
+            G.n_items = (uint32_t)lists[g].size();
+            G.grid = G.n_items;
+            G.lds = maxlds[g];
+            if (!G.n_items) continue;
+            if (int rc = set_lds_attr(plain_kernel(g), G.lds)) return bail(rc);
+            if (int rc = dalloc(&G.d_list, G.n_items)) return bail(rc);
+            HIPCHK(hipMemcpy(G.d_list, lists[g].data(), sizeof(uint32_t) * G.n_items, hipMemcpyHostToDevice));
+        }
+    } else {
+        // slots refer to internal chain indices already (pack assigned them in order)
+        if (int rc = dalloc(&e->d_slots, slots.size())) return bail(rc);
+        HIPCHK(hipMemcpy(e->d_slots, slots.data(), sizeof(DevSlot) * slots.size(), hipMemcpyHostToDevice));
+        for (int g = 0; g < 2; ++g) {
+            Group &G = e->grp[g];
+            G.gx = g;
+            std::vector<DevBundle> &bv = bundles[g];
+            // longest-processing-time first
+            std::stable_sort(bv.begin(), bv.end(), [&](const DevBundle &a, const DevBundle &b) {
+                return bundle_cost(a, e->hch, slots) > bundle_cost(b, e->hch, slots);
+            });
+            G.n_items = (uint32_t)bv.size();
+            if (!G.n_items) continue;
+            size_t maxd = 0;
+            for (auto &b : bv) maxd = std::max<size_t>(maxd, b.lds_doubles);
+            G.lds = g ? 0 : maxd * sizeof(double);
+            const void *fn = batched_kernel(e->form == JMM_FORM_BATCHED_FP32, e->T, g);
+            if (int rc = set_lds_attr(fn, G.lds)) return bail(rc);
+            int occ = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, e->T, G.lds));
+            if (occ < 1) occ = 1;
+            G.grid = std::min<uint32_t>(G.n_items, (uint32_t)(occ * n_cu));
+            if (int rc = dalloc(&G.d_bundles, G.n_items)) return bail(rc);
+            HIPCHK(hipMemcpy(G.d_bundles, bv.data(), sizeof(DevBundle) * G.n_items, hipMemcpyHostToDevice));
+        }
+    }
+
+    if (e->use_graph) {
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_block(e, e->block_sweeps);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (rc) return bail(rc);
+        if (ce != hipSuccess) return bail(fail(JMM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ce)));
+        e->graph = g;
+        HIPCHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    *out = e;
+    return 0;
+}
+
+int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps)
+{
+    if (!e) return fail(JMM_EINVAL, "jmm_run_sweeps: null engine");
+    if ((uint64_t)e->write_pos + n_sweeps > e->cap)
+        return fail(JMM_EFULL, "sample buffer holds %u sweeps, %u pending: read samples first", e->cap, e->write_pos);
+    HIPCHK(hipSetDevice(e->device));
+    while (n_sweeps > 0) {
+        const uint32_t m = std::min(n_sweeps, e->block_sweeps);
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        e->ev.push_back({a, b});
+        HIPCHK(hipEventRecord(a, e->stream));
+        if (e->gexec && m == e->block_sweeps) {
+            HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+        } else {
+            if (int rc = launch_block(e, m)) return rc;
+        }
+        HIPCHK(hipEventRecord(b, e->stream));
+        e->ev_sweeps += m;
+        e->write_pos += m;
+        n_sweeps -= m;
+    }
+    return 0;
+}
+
+int jmm_synchronize(jmm_engine *e)
+{
+    if (!e) return fail(JMM_EINVAL, "jmm_synchronize: null engine");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *sweeps, int reset)
+{
+    if (!e) return fail(JMM_EINVAL, "jmm_kernel_time: null engine");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    double tot = 0.0;
+    for (auto &p : e->ev) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, p.first, p.second));
+        tot += t;
+    }
+    if (launches) *launches = e->ev.size();
+    if (ms) *ms = tot;
+    if (sweeps) *sweeps = e->ev_sweeps;
+    if (reset) {
+        for (auto &p : e->ev) {
+            hipEventDestroy(p.first);
+            hipEventDestroy(p.second);
+        }
+        e->ev.clear();
+        e->ev_sweeps = 0;
+    }
+    return 0;
+}
+
+int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out)
+{
+    if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_samples: null argument");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const uint32_t n = e->write_pos;
+    if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_samples: %u sweeps pending, room for %u", n, max_sweeps);
+    if (n && !out) return fail(JMM_EINVAL, "jmm_read_samples: out is NULL");
+    const size_t rows = (size_t)n * e->n;
+    std::vector<double> U(rows), L(rows);
+    std::vector<uint2> A(rows);
+    if (rows) {
+        HIPCHK(hipMemcpy(U.data(), e->d_sU, rows * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(L.data(), e->d_sL, rows * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(A.data(), e->d_sA, rows * sizeof(uint2), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t s = 0; s < n; ++s)
+        for (int k = 0; k < e->n; ++k) {
+            const size_t r = (size_t)s * e->n + k;
+            jmm_sample &o = out[(size_t)e->ext_of_int[k] * max_sweeps + s];
+            o.U = U[r];
+            o.L = L[r];
+            o.acc_d = A[r].x;
+            o.acc_v = A[r].y;
+        }
+    e->write_pos = 0;
+    Ctrl zc[2] = {};
+    HIPCHK(hipMemcpy(e->d_ctrl, zc, sizeof zc, hipMemcpyHostToDevice));
+    *n_out = n;
+    return 0;
+}
+
+int jmm_read_accept_log(jmm_engine *e, uint8_t *out, uint32_t max_sweeps, uint32_t *n_out)
+{
+    if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_accept_log: null argument");
+    if (!e->d_log) return fail(JMM_ESTATE, "engine was created without log_accepts");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const uint32_t n = e->write_pos;
+    if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_accept_log: %u sweeps pending, room for %u", n, max_sweeps);
+    for (int k = 0; k < e->n && n; ++k) {
+        const uint32_t c = e->ext_of_int[k];
+        const uint64_t N = e->specs[c].N;
+        HIPCHK(hipMemcpy(out + e->ext_off[c] * max_sweeps, e->d_log + e->hch[k].log_off,
+                         (size_t)n * N, hipMemcpyDeviceToHost));
+    }
+    *n_out = n;
+    return 0;
+}
+
+int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uint64_t *sweep)
+{
+    if (!e || chain < 0 || chain >= e->n) return fail(JMM_EINVAL, "jmm_get_state: bad chain");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const uint32_t k = e->int_of_ext[chain];
+    if (x) HIPCHK(hipMemcpy(x, e->d_x + e->hch[k].x_off, sizeof(double) * e->hch[k].N, hipMemcpyDeviceToHost));
+    DevState st;
+    HIPCHK(hipMemcpy(&st, e->d_states + k, sizeof st, hipMemcpyDeviceToHost));
+    if (L) *L = st.L;
+    if (U) *U = st.U;
+    if (sweep) *sweep = st.sweep;
+    return 0;
+}
+
+int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t sweep)
+{
+    if (!e || chain < 0 || chain >= e->n || !x) return fail(JMM_EINVAL, "jmm_set_state: bad argument");
+    const jmm_chain_spec &s = e->specs[chain];
+    const uint32_t k = e->int_of_ext[chain];
+    const DevChain &d = e->hch[k];
+    const bool per = s.boundary == JMM_PERIODIC;
+    if (!(L > 0.0) || (per && !(L > d.Lmin))) return fail(JMM_EINVAL, "jmm_set_state: bad length %g", L);
+    for (uint64_t i = 0; i < s.N; ++i) {
+        if (!std::isfinite(x[i])) return fail(JMM_EINVAL, "jmm_set_state: position %llu not finite", (unsigned long long)i);
+        if (!per && std::fabs(x[i]) > L / 2.0) return fail(JMM_EINVAL, "jmm_set_state: position %llu outside the walls", (unsigned long long)i);
+        if (i + 1 < s.N && !(x[i + 1] - x[i] >= d.gmin))
+            return fail(JMM_EINVAL, "jmm_set_state: positions %llu,%llu unsorted or inside the hard core", (unsigned long long)i, (unsigned long long)i + 1);
+    }
+    if (per && s.N >= 2 && !(x[0] + L - x[s.N - 1] >= d.gmin))
+        return fail(JMM_EINVAL, "jmm_set_state: wrap-around pair inside the hard core");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    DevState st;
+    HIPCHK(hipMemcpy(&st, e->d_states + k, sizeof st, hipMemcpyDeviceToHost));
+    st.L = L;
+    st.U = host_energy(s, d.pot, x, L, d.w);
+    st.sweep = sweep;
+    HIPCHK(hipMemcpy(e->d_x + d.x_off, x, sizeof(double) * s.N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_states + k, &st, sizeof st, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int jmm_perturb_energy(jmm_engine *e, int chain, double delta)
+{
+    if (!e || chain < 0 || chain >= e->n) return fail(JMM_EINVAL, "jmm_perturb_energy: bad chain");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const uint32_t k = e->int_of_ext[chain];
+    DevState st;
+    HIPCHK(hipMemcpy(&st, e->d_states + k, sizeof st, hipMemcpyDeviceToHost));
+    st.U += delta;
+    HIPCHK(hipMemcpy(e->d_states + k, &st, sizeof st, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
+{
+    if (!e || chain < 0 || chain >= e->n || !out) return fail(JMM_EINVAL, "jmm_get_counters: bad argument");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const uint32_t k = e->int_of_ext[chain];
+    DevState st;
+    HIPCHK(hipMemcpy(&st, e->d_states + k, sizeof st, hipMemcpyDeviceToHost));
+    out->sweeps = st.sweep;
+    out->trials = st.cnt[0];
+    out->acc_d = st.cnt[1];
+    out->wall_rejects = st.cnt[2];
+    out->core_rejects = st.cnt[3];
+    out->pair_terms = st.cnt[4];
+    out->vol_trials = st.cnt[5];
+    out->acc_v = st.cnt[6];
+    out->recomputes = st.cnt[7];
+    out->max_drift = st.max_drift;
+    out->U = st.U;
+    out->L = st.L;
+    return 0;
+}
+
+static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32)
+{
+    HIPCHK(hipSetDevice(e->device));
+    if (!e->d_obs)
+        if (int rc = dalloc(&e->d_obs, (size_t)e->n * 6)) return rc;
+    if (!e->d_Ltmp)
+        if (int rc = dalloc(&e->d_Ltmp, (size_t)e->n)) return rc;
+    if (!e->d_all) {
+        if (int rc = dalloc(&e->d_all, (size_t)e->n)) return rc;
+        std::vector<uint32_t> all(e->n);
+        std::iota(all.begin(), all.end(), 0u);
+        HIPCHK(hipMemcpy(e->d_all, all.data(), sizeof(uint32_t) * e->n, hipMemcpyHostToDevice));
+    }
+    if (grad && !e->d_grad)
+        if (int rc = dalloc(&e->d_grad, e->total_N)) return rc;
+    HIPCHK(launch_lengths(e->d_states, e->d_Ltmp, (uint32_t)e->n, e->stream));
+    ObsArgs a{};
+    a.chains = e->d_chains;
+    a.x = e->d_x;
+    a.L = e->d_Ltmp;
+    a.list = e->d_all;
+    a.obs = e->d_obs;
+    a.grad = grad ? e->d_grad : nullptr;
+    a.n = (uint32_t)e->n;
+    a.standalone = 0;
+    HIPCHK(launch_observables(a, fp32, (uint32_t)e->n, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<double> ho((size_t)e->n * 6);
+    HIPCHK(hipMemcpy(ho.data(), e->d_obs, sizeof(double) * ho.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < e->n; ++k)
+        for (int m = 0; m < 6; ++m) obs[(size_t)e->ext_of_int[k] * 6 + m] = ho[(size_t)k * 6 + m];
+    if (grad) {
+        std::vector<double> hg(e->total_N);
+        HIPCHK(hipMemcpy(hg.data(), e->d_grad, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < e->n; ++k) {
+            const uint32_t c = e->ext_of_int[k];
+            std::memcpy(grad + e->ext_off[c], hg.data() + e->hch[k].x_off, sizeof(double) * e->hch[k].N);
+        }
+    }
+    return 0;
+}
+
+int jmm_observables(jmm_engine *e, double *obs, double *grad)
+{
+    if (!e || !obs) return fail(JMM_EINVAL, "jmm_observables: null argument");
+    return observables_impl(e, obs, grad, e->form == JMM_FORM_BATCHED_FP32);
+}
+
+int jmm_full_energy(jmm_engine *e, double *U_out)
+{
+    if (!e || !U_out) return fail(JMM_EINVAL, "jmm_full_energy: null argument");
+    std::vector<double> obs((size_t)e->n * 6);
+    if (int rc = observables_impl(e, obs.data(), nullptr, 0)) return rc;
+    for (int c = 0; c < e->n; ++c) U_out[c] = obs[(size_t)c * 6];
+    return 0;
+}
+
+int jmm_pair_observables(const jmm_chain_spec *spec, int n_configs, const double *x_dev,
+                         const double *L_dev, double *obs_dev, double *grad_dev, int precision,
+                         void *stream)
+{
+    if (!spec || n_configs <= 0 || !x_dev || !L_dev || !obs_dev)
+        return fail(JMM_EINVAL, "jmm_pair_observables: null or empty argument");
+    if (precision != 32 && precision != 64) return fail(JMM_EINVAL, "jmm_pair_observables: precision must be 32 or 64");
+    jmm_chain_spec s = *spec;
+    s.L0 = std::isfinite(s.L0) && s.L0 > 0 ? s.L0 : 1.0;
+    if (s.N < 1 || s.potential < 0 || s.potential > 3 || !(s.cutoff > 0.0))
+        return fail(JMM_EINVAL, "jmm_pair_observables: bad potential description");
+    ObsArgs a{};
+    a.one = make_chain(s);
+    a.x = x_dev;
+    a.L = L_dev;
+    a.obs = obs_dev;
+    a.grad = grad_dev;
+    a.n = (uint32_t)n_configs;
+    a.standalone = 1;
+    HIPCHK(launch_observables(a, precision == 32, (uint32_t)n_configs, (hipStream_t)stream));
+    return 0;
+}
+
+}  // extern "C"

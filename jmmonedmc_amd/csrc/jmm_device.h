@@ -1,0 +1,219 @@
+// jmm_device.h -- arithmetic shared by the gfx950 kernels and the engine's
+// host set-up code: Philox4x32-10, 53-bit uniforms, the deterministic exp of
+// the log-length move, and the pair potentials in two forms:
+//   phi64  FP64, one IEEE rounding per operation in a fixed order (the file
+//          is compiled with -ffp-contract=off), so positions, per-pair energies
+//          and per-trial energy changes are reproducible bit for bit;
+//   phi32  FP32 pair arithmetic on an FP64-formed separation (the caller
+//          subtracts positions in FP64 and decides the cutoff in FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#define JMM_HD __host__ __device__ __forceinline__
+
+namespace jmm {
+
+enum Kind : int { POT_LJ = 0, POT_HARMONIC = 1, POT_HRSW = 2, POT_YUKAWA = 3 };
+
+// Philox4x32-10 (Salmon et al., SC'11).  ctr = (slot, sweep, chain_id, 0),
+// key = seed.  Replaces the reference's GSL taus2 stream
+// (src/jmmMCState.cpp:779-781, draws at :1762-1763, :1370, :2252).
+JMM_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                          uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// 53-bit uniform in [0, 1) from the high 27 bits of a and 26 bits of b.
+JMM_HD double u53(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// exp for the log-length move's scale factor, built only from IEEE
+// add/mul/floor so host and device agree bit for bit: Cody-Waite reduction
+// x = k ln2 + r with a two-constant ln2, degree-13 Taylor polynomial in
+// Horner form, exact 2^k scaling.
+JMM_HD double det_exp(double x)
+{
+    if (x > 709.0) return INFINITY;
+    if (x < -708.0) return 0.0;
+    const double inv_ln2 = 1.4426950408889634;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double kd = floor(x * inv_ln2 + 0.5);
+    const double r = (x - kd * ln2_hi) - kd * ln2_lo;
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    const int64_t k = (int64_t)kd;
+    union {
+        uint64_t u;
+        double d;
+    } sc;
+    sc.u = (uint64_t)(k + 1023) << 52;
+    return p * sc.d;
+}
+
+// Per-chain potential constants, prepared once on the host.
+struct Pot {
+    double rc;     // cutoff (inclusive for LJ/Yukawa, strict for harmonic/HRSW)
+    double shift;  // phi(rc) for the shifted form, else 0
+    double e4;     // LJ: 4 eps
+    double sig6;   // LJ: sigma^6
+    double p0, p1, p2;
+    float rcf, shiftf, e4f, sig6f, p0f, p1f, p2f, pad;
+};
+
+// FP64 pair energy, operation for operation the oracle's sequence
+// (LJ: src/pot.cpp:47-56 generalised to eps, sigma; harmonic:
+// src/pot.cpp:116-131).
+template <int KIND>
+JMM_HD double phi64(const Pot &q, double d)
+{
+    if (KIND == POT_LJ) {
+        if (!(d <= q.rc)) return 0.0;
+        const double d3 = d * d * d;
+        const double i6 = (1.0 / (d3 * d3)) * q.sig6;
+        const double i12 = i6 * i6;
+        return (q.e4 * i12 - q.e4 * i6) - q.shift;
+    } else if (KIND == POT_HARMONIC) {
+        if (d <= 0.0) return 10e10;
+        if (d < q.rc) {
+            const double m = d - 1.0;
+            return m * m;
+        }
+        return 0.0;
+    } else if (KIND == POT_HRSW) {
+        if (d < q.p0) return INFINITY;
+        return d < q.p2 ? -q.p1 : 0.0;
+    } else {
+        if (d < q.p2) return INFINITY;
+        if (!(d <= q.rc)) return 0.0;
+        return (q.p0 * (exp(-q.p1 * d) / d)) - q.shift;
+    }
+}
+
+// Inside-the-cutoff test in FP64 (the FP32 form uses it so that both forms
+// sum the same pairs).
+template <int KIND>
+JMM_HD bool inside64(const Pot &q, double d)
+{
+    if (KIND == POT_HARMONIC || KIND == POT_HRSW) return d < (KIND == POT_HRSW ? q.p2 : q.rc);
+    return d <= q.rc;
+}
+
+// Hardware reciprocal and exponential on the device (v_rcp_f32, v_exp_f32);
+// plain library forms on the host.
+JMM_HD float frcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+JMM_HD float fexp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __expf(x);
+#else
+    return expf(x);
+#endif
+}
+
+// FP32 pair energy of an FP64-formed separation d whose cutoff test `in` was
+// made in FP64.  Hard cores are handled before any pair is evaluated.
+template <int KIND>
+JMM_HD float phi32(const Pot &q, float d, bool in)
+{
+    if (!in) return 0.0f;
+    if (KIND == POT_LJ) {
+        const float ir2 = frcp(d * d);
+        const float s6 = ir2 * ir2 * ir2 * q.sig6f;
+        return q.e4f * s6 * (s6 - 1.0f) - q.shiftf;
+    } else if (KIND == POT_HARMONIC) {
+        if (d <= 0.0f) return 10e10f;
+        const float m = d - 1.0f;
+        return m * m;
+    } else if (KIND == POT_HRSW) {
+        return -q.p1f;
+    } else {
+        return q.p0f * fexp(-q.p1f * d) * frcp(d) - q.shiftf;
+    }
+}
+
+// Pair observables for the backward kernel: E, W = -d phi', dE/dtheta0,
+// dE/dtheta1, dW/dtheta0, dW/dtheta1 (LJ theta = (eps, sigma); Yukawa
+// theta = (A, kappa); hard-core deltas excluded).  `sh` holds the shifted
+// form's values at the cutoff (E, dE/dtheta0, dE/dtheta1), zero otherwise.
+template <int KIND, typename R>
+JMM_HD void pair_obs(const Pot &q, R d, const R sh[3], R o[6])
+{
+    for (int k = 0; k < 6; ++k) o[k] = R(0);
+    if (KIND == POT_LJ) {
+        const R d3 = d * d * d;
+        const R i6 = (R(1) / (d3 * d3)) * R(q.sig6);
+        const R i12 = i6 * i6;
+        const R e4 = R(q.e4);
+        const R a12 = e4 * i12, a6 = e4 * i6;
+        const R sig = R(q.p1);
+        o[0] = (a12 - a6) - sh[0];
+        o[1] = R(12) * a12 - R(6) * a6;
+        o[2] = (R(4) * i12 - R(4) * i6) - sh[1];
+        o[3] = o[1] / sig - sh[2];
+        o[4] = R(12) * (R(4) * i12) - R(6) * (R(4) * i6);
+        o[5] = (R(144) * a12 - R(36) * a6) / sig;
+    } else if (KIND == POT_YUKAWA) {
+        const R A = R(q.p0), k = R(q.p1);
+        const R e = exp(-k * d);
+        const R ed = e / d;
+        const R E = A * ed;
+        o[0] = E - sh[0];
+        o[1] = E * (k * d + R(1));
+        o[2] = ed - sh[1];
+        o[3] = -A * e - sh[2];
+        o[4] = ed * (k * d + R(1));
+        o[5] = -k * d * d * E;
+    } else if (KIND == POT_HARMONIC) {
+        const R m = d - R(1);
+        o[0] = m * m;
+        o[1] = R(-2) * d * m;
+    } else {
+        o[0] = R(-q.p1);
+        o[2] = R(-1);
+    }
+}
+
+}  // namespace jmm

@@ -1,0 +1,97 @@
+// kernels.h -- device data layout and the gfx950 kernels of the engine.
+//
+// HBM layout (one engine):
+//   x        double[sum N]     chain positions, sorted, chains in bundle order
+//   chains   DevChain[n]       read-only per-chain constants
+//   states   DevState[n]       L, U, sweep counter, cumulative counters
+//   sU, sL   double[cap][n]    per-sweep samples, one row per sweep (coalesced)
+//   sA       uint2[cap][n]     accepted displacements / box-length trials
+//   bundles  DevBundle[m]      work items: chains packed into one workgroup
+//   slots    DevSlot[...]      chain -> lane range and LDS offset in a bundle
+//
+// Sweep schedule (shared with the oracle): chain c's N particles form B_c
+// blocks (B_c a power of two, every block at least M_min particles); in phase
+// t every block updates its t-th particle.  Trials in one phase are at least
+// M_min apart in index order, and M_min is chosen so that no two of them can
+// interact or read each other's particle (hard core g_min, cutoff rc, step
+// delta: M_min = floor((rc + 2 delta)/g_min) + 1; or NBN + 1).  A phase is
+// therefore equivalent to running its trials one after another in any order.
+#pragma once
+
+#include "jmm_device.h"
+
+namespace jmm {
+
+enum : uint32_t { F_PERIODIC = 1u, F_NPT = 2u, F_VOLLOG = 4u };
+
+struct DevChain {
+    uint64_t seed;
+    uint64_t x_off;   // first position of this chain in x[]
+    uint64_t log_off; // first byte of this chain's accept log
+    uint32_t N, B, w, smax;
+    uint32_t chain_id, eci, flags, kind;
+    double beta, P, max_step, max_dv, gmin, Lmin;
+    double sh[3];  // shifted form at the cutoff: E, dE/dtheta0, dE/dtheta1
+    Pot pot;
+    float betaf, pad[3];
+};
+
+struct DevState {
+    double L, U;
+    uint64_t sweep;
+    uint64_t cnt[8];  // trials, acc_d, wall, core, pair_terms, vol_trials, acc_v, recomputes
+    double max_drift;
+};
+
+struct DevBundle {
+    uint32_t slot0, nslots, lanes, smax;
+    uint32_t lds_doubles, kind, global_x, pad;
+    uint64_t x_off;
+};
+
+struct DevSlot {
+    uint32_t chain, lane_off, B, lds_off;
+};
+
+struct Ctrl {
+    uint32_t queue, sample_off, next_off, pad;
+};
+
+struct SweepArgs {
+    const DevChain *chains;
+    DevState *states;
+    double *x;
+    const DevBundle *bundles;
+    const DevSlot *slots;
+    const uint32_t *plain_list;  // plain form: chains of this launch
+    Ctrl *ctrl;
+    double *sU, *sL;
+    uint2 *sA;
+    uint8_t *acclog;
+    uint32_t n_items, n_sweeps, n_chains, cap;
+};
+
+struct ObsArgs {
+    const DevChain *chains;  // engine form (nullable)
+    DevChain one;            // standalone form
+    const double *x;
+    const double *L;
+    const uint32_t *list;    // engine form: chains of this launch
+    double *obs;
+    double *grad;
+    uint32_t n;
+    uint32_t standalone;
+};
+
+}  // namespace jmm
+
+namespace jmm {
+hipError_t launch_begin(Ctrl *c, uint32_t n, hipStream_t s);
+const void *batched_kernel(int fp32, int T, int gx);
+hipError_t launch_batched(const SweepArgs &a, int fp32, int T, int gx, uint32_t grid, size_t lds,
+                          hipStream_t s);
+const void *plain_kernel(int gx);
+hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, hipStream_t s);
+hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s);
+hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s);
+}  // namespace jmm

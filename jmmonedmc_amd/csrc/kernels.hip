@@ -1,0 +1,813 @@
+// kernels.hip -- gfx950 kernels of the Metropolis engine.
+//
+// Compiled with -ffp-contract=off: the FP64 paths round once per operation in
+// the oracle's order, which makes positions and per-trial energy changes
+// reproducible bit for bit (only sums over lanes differ in their order).
+//
+//   k_sweep_batched  persistent, work-queue driven: one workgroup takes a
+//                    bundle of chains, keeps their sorted positions in LDS,
+//                    one lane per block of particles, phases separated by
+//                    workgroup barriers.  FP64 or FP32 pair arithmetic.
+//   k_sweep_plain    one wavefront per chain; trials one at a time, the
+//                    energy change a wavefront reduction over the neighbour
+//                    window (lanes 0-31 left partners, 32-63 right).
+//   k_observables    U, virial, dU/dtheta, dW/dtheta and dU/dx per
+//                    configuration (the differentiation path).
+//   k_begin          resets a work queue and advances the sample row.
+#include "kernels.h"
+
+namespace jmm {
+
+__device__ __forceinline__ bool nbpos(const double *X, int64_t j, int64_t N, bool per, double L,
+                                      double &xj)
+{
+    if (j >= 0 && j < N) {
+        xj = X[j];
+        return true;
+    }
+    if (!per) return false;
+    xj = (j < 0) ? X[j + N] - L : X[j - N] + L;
+    return true;
+}
+
+// Energy change for moving the particle at xi (index i) to xn.  Partners:
+// left side nearest first, then right side, stopping at the first partner
+// beyond the cutoff from both positions or at index distance w.  Restates the
+// partner sums of qad2 (src/jmmMCState.cpp:1213-1348), whose far pairs
+// contribute exactly zero.
+template <int KIND>
+__device__ __forceinline__ double delta_e64(const Pot &q, const double *X, int64_t i, int64_t N,
+                                            bool per, double L, uint32_t w, double xi, double xn,
+                                            uint32_t &terms)
+{
+    double dE = 0.0;
+    for (uint32_t k = 1; k <= w; ++k) {
+        double xj;
+        if (!nbpos(X, i - (int64_t)k, N, per, L, xj)) break;
+        const double dold = xi - xj, dnew = xn - xj;
+        if (dold > q.rc && dnew > q.rc) break;
+        dE += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
+        terms += 2;
+    }
+    for (uint32_t k = 1; k <= w; ++k) {
+        double xj;
+        if (!nbpos(X, i + (int64_t)k, N, per, L, xj)) break;
+        const double dold = xj - xi, dnew = xj - xn;
+        if (dold > q.rc && dnew > q.rc) break;
+        dE += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
+        terms += 2;
+    }
+    return dE;
+}
+
+// FP32 form: separations formed in FP64 (positions are FP64), cutoff decided
+// in FP64, pair energies in FP32, Kahan-compensated sum.
+template <int KIND>
+__device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64_t i, int64_t N,
+                                            bool per, double L, uint32_t w, double xi, double xn,
+                                            uint32_t &terms)
+{
+    float s = 0.0f, c = 0.0f;
+    for (uint32_t k = 1; k <= w; ++k) {
+        double xj;
+        if (!nbpos(X, i - (int64_t)k, N, per, L, xj)) break;
+        const double dold = xi - xj, dnew = xn - xj;
+        if (dold > q.rc && dnew > q.rc) break;
+        const float t = phi32<KIND>(q, (float)dnew, inside64<KIND>(q, dnew)) -
+                        phi32<KIND>(q, (float)dold, inside64<KIND>(q, dold));
+        const float y = t - c;
+        const float tt = s + y;
+        c = (tt - s) - y;
+        s = tt;
+        terms += 2;
+    }
+    for (uint32_t k = 1; k <= w; ++k) {
+        double xj;
+        if (!nbpos(X, i + (int64_t)k, N, per, L, xj)) break;
+        const double dold = xj - xi, dnew = xj - xn;
+        if (dold > q.rc && dnew > q.rc) break;
+        const float t = phi32<KIND>(q, (float)dnew, inside64<KIND>(q, dnew)) -
+                        phi32<KIND>(q, (float)dold, inside64<KIND>(q, dold));
+        const float y = t - c;
+        const float tt = s + y;
+        c = (tt - s) - y;
+        s = tt;
+        terms += 2;
+    }
+    return (double)s - (double)c;
+}
+
+// Pair-by-pair energy change of scaling every coordinate by sc (box-length
+// trial), over the pairs whose left member is in [p0, p1).  Sets bad when a
+// scaled nearest pair falls inside the hard core.  Restates fav's trial
+// energy (src/jmmMCState.cpp:2175-2235) as a difference per pair.
+template <typename Real, int KIND>
+__device__ __forceinline__ double scaled_delta(const Pot &q, double gmin, const double *X, int64_t N,
+                                               bool per, double L, double Lp, double sc,
+                                               uint32_t w, uint32_t p0, uint32_t p1, uint32_t stride,
+                                               uint32_t &bad)
+{
+    double part = 0.0;
+    float s = 0.0f, c = 0.0f;
+    for (uint32_t p = p0; p < p1 && !bad; p += stride) {
+        const double xi = X[p], xis = xi * sc;
+        for (uint32_t k = 1; k <= w; ++k) {
+            const int64_t j = (int64_t)p + k;
+            double xj, xjs;
+            if (j < N) {
+                xj = X[j];
+                xjs = xj * sc;
+            } else if (per) {
+                xj = X[j - N] + L;
+                xjs = X[j - N] * sc + Lp;
+            } else {
+                break;
+            }
+            const double d0 = xj - xi, d1 = xjs - xis;
+            if (k == 1 && d1 < gmin) {
+                bad = 1;
+                break;
+            }
+            if (d0 > q.rc && d1 > q.rc) break;
+            if (sizeof(Real) == 8) {
+                part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
+            } else {
+                const float t = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
+                                phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
+                const float y = t - c;
+                const float tt = s + y;
+                c = (tt - s) - y;
+                s = tt;
+            }
+        }
+    }
+    if (sizeof(Real) == 8) return part;
+    return (double)s - (double)c;
+}
+
+// FP64 full energy of the pairs whose left member is in [p0, p1) (ECheck's
+// recompute, src/jmmMCState.cpp:1973-1993).
+template <int KIND>
+__device__ __forceinline__ double energy_part(const Pot &q, const double *X, int64_t N, bool per,
+                                              double L, uint32_t w, uint32_t p0, uint32_t p1,
+                                              uint32_t stride)
+{
+    double U = 0.0;
+    for (uint32_t p = p0; p < p1; p += stride) {
+        const double xi = X[p];
+        for (uint32_t k = 1; k <= w; ++k) {
+            const int64_t j = (int64_t)p + k;
+            double xj;
+            if (j < N) xj = X[j];
+            else if (per) xj = X[j - N] + L;
+            else break;
+            const double d = xj - xi;
+            if (d > q.rc) break;
+            U += phi64<KIND>(q, d);
+        }
+    }
+    return U;
+}
+
+// Acceptance of a box-length trial (src/jmmMCState.cpp:2249-2255).
+__device__ __forceinline__ bool vol_accept(double dU, double dL, double lns, double P, double beta,
+                                           double jac, double u)
+{
+    const double bf = exp(-beta * (dU + P * dL) + jac * lns);
+    return bf >= 1.0 || bf > u;
+}
+
+struct VolProp {
+    double sc, Lp, lns, jac, dL, u;
+    bool valid;
+};
+
+__device__ __forceinline__ VolProp vol_propose(const DevChain &ch, double L, uint64_t sweep)
+{
+    uint32_t r[4];
+    philox4x32_10(0xFFFFFFFFu, (uint32_t)sweep, ch.chain_id, 0u, (uint32_t)ch.seed,
+                  (uint32_t)(ch.seed >> 32), r);
+    const double a = u53(r[0], r[1]);
+    VolProp v;
+    v.u = u53(r[2], r[3]);
+    const double dv = ((a - 0.5) * 2.0) * ch.max_dv;
+    if (!(ch.flags & F_VOLLOG)) {
+        v.Lp = L + dv;
+        v.sc = v.Lp / L;
+        v.lns = log(v.sc);
+        v.jac = (double)ch.N;
+        v.dL = dv;
+    } else {
+        v.sc = det_exp(dv);
+        v.Lp = L * v.sc;
+        v.lns = dv;
+        v.jac = (double)ch.N + 1.0;
+        v.dL = v.Lp - L;
+    }
+    v.valid = v.Lp > 0.0 && (!(ch.flags & F_PERIODIC) || v.Lp > ch.Lmin);
+    return v;
+}
+
+
+// Sum over aligned power-of-two lane segments of size seg (seg <= T).  Every
+// lane of a segment receives the same bits: the butterfly is symmetric in its
+// operands, and the cross-wave stage reads the wave totals in a fixed order.
+// rd/ri: flat LDS scratch of (T/64)*ND doubles and (T/64)*NI words.
+template <int T, int ND, int NI>
+__device__ __forceinline__ void seg_allreduce(double (&v)[ND], uint32_t (&k)[NI], uint32_t seg,
+                                              double *rd, uint32_t *ri)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int m = 0; m < ND; ++m) {
+            const double p = __shfl_xor(v[m], off);
+            if ((uint32_t)off < seg) v[m] += p;
+        }
+#pragma unroll
+        for (int m = 0; m < NI; ++m) {
+            const uint32_t p = __shfl_xor(k[m], off);
+            if ((uint32_t)off < seg) k[m] += p;
+        }
+    }
+    if (T > 64) {
+        const uint32_t tid = threadIdx.x, wv = tid >> 6;
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int m = 0; m < ND; ++m) rd[wv * ND + m] = v[m];
+#pragma unroll
+            for (int m = 0; m < NI; ++m) ri[wv * NI + m] = k[m];
+        }
+        __syncthreads();
+        if (seg > 64) {
+            const uint32_t w0 = (tid & ~(seg - 1)) >> 6, nw = seg >> 6;
+#pragma unroll
+            for (int m = 0; m < ND; ++m) {
+                double acc = 0.0;
+                for (uint32_t q = 0; q < nw; ++q) acc += rd[(w0 + q) * ND + m];
+                v[m] = acc;
+            }
+#pragma unroll
+            for (int m = 0; m < NI; ++m) {
+                uint32_t acc = 0;
+                for (uint32_t q = 0; q < nw; ++q) acc += ri[(w0 + q) * NI + m];
+                k[m] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Full 64-lane sum, identical in every lane.
+template <typename V>
+__device__ __forceinline__ V wave_sum(V v)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// ------------------------------------------------------------ batched form
+
+template <int T>
+struct BatchedScratch {
+    double rd[(T / 64) * 2];
+    uint32_t ri[(T / 64) * 4];
+    uint32_t item;
+};
+
+template <typename Real, int KIND, int T, bool GX>
+__device__ void run_bundle(const SweepArgs &a, const DevBundle &bd, double *smem,
+                           BatchedScratch<T> &sc, uint32_t soff)
+{
+    const uint32_t tid = threadIdx.x;
+    // lane -> slot: the largest slot whose first lane is <= tid
+    uint32_t lo = 0, hi = bd.nslots - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a.slots[bd.slot0 + mid].lane_off <= tid) lo = mid;
+        else hi = mid - 1;
+    }
+    const DevSlot me = a.slots[bd.slot0 + lo];
+    const bool act = tid >= me.lane_off && tid < me.lane_off + me.B;
+    const uint32_t ic = me.chain;
+
+    double *X;
+    if (GX) {
+        X = a.x + bd.x_off;
+    } else {
+        const double *xg = a.x + bd.x_off;
+        for (uint32_t k = tid; k < bd.lds_doubles; k += T) smem[k] = xg[k];
+        X = smem + me.lds_off;
+    }
+    __syncthreads();
+
+    const DevChain &ch = a.chains[ic];
+    const Pot q = ch.pot;
+    const int64_t N = ch.N;
+    const uint32_t w = ch.w;
+    const bool per = (ch.flags & F_PERIODIC) != 0;
+    const bool npt = (ch.flags & F_NPT) != 0;  // uniform within a bundle
+    const double beta = ch.beta, gmin = ch.gmin, max_step = ch.max_step;
+    const float betaf = ch.betaf;
+    const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+
+    const DevState st = a.states[ic];
+    double L = st.L, U = st.U, mdrift = st.max_drift;
+    uint64_t sweep = st.sweep;
+
+    const uint32_t B = me.B;
+    const uint32_t r = tid - me.lane_off;
+    const uint32_t qb = (uint32_t)N / B, rem = (uint32_t)N % B;
+    const uint32_t size = act ? qb + (r < rem ? 1u : 0u) : 0u;
+    const uint32_t start = act ? r * qb + (r < rem ? r : rem) : 0u;
+    const uint32_t seg = act ? B : 1u;
+    const bool leader = act && r == 0;
+
+    uint32_t n_tr = 0, n_wall = 0, n_core = 0, n_terms = 0;
+    uint32_t acc_total = 0, n_vt = 0, n_av = 0, n_rc = 0;
+
+    for (uint32_t sw = 0; sw < a.n_sweeps; ++sw, ++sweep) {
+        double dUs = 0.0;
+        uint32_t acc = 0;
+        for (uint32_t t = 0; t < bd.smax; ++t) {
+            if (t < size) {
+                const uint32_t i = start + t;
+                uint32_t rr[4];
+                philox4x32_10(i, (uint32_t)sweep, ch.chain_id, 0u, key0, key1, rr);
+                const double a1 = u53(rr[0], rr[1]);
+                const double u = u53(rr[2], rr[3]);
+                const double md = ((a1 - 0.5) * 2.0) * max_step;
+                const double xi = X[i];
+                const double xn = xi + md;
+                ++n_tr;
+                bool ok = true;
+                if (!per && fabs(xn) > L / 2.0) {
+                    ok = false;
+                    ++n_wall;
+                } else if (N >= 2) {
+                    double xl, xr;
+                    if ((nbpos(X, (int64_t)i - 1, N, per, L, xl) && (xn - xl) < gmin) ||
+                        (nbpos(X, (int64_t)i + 1, N, per, L, xr) && (xr - xn) < gmin)) {
+                        ok = false;
+                        ++n_core;
+                    }
+                }
+                bool accd = false;
+                if (ok) {
+                    double dE;
+                    if (sizeof(Real) == 8)
+                        dE = delta_e64<KIND>(q, X, i, N, per, L, w, xi, xn, n_terms);
+                    else
+                        dE = delta_e32<KIND>(q, X, i, N, per, L, w, xi, xn, n_terms);
+                    if (dE <= 0.0) accd = true;
+                    else if (sizeof(Real) == 8) accd = exp(-dE * beta) > u;
+                    else accd = (double)fexp(-betaf * (float)dE) > u;
+                    if (accd) {
+                        X[i] = xn;
+                        dUs += dE;
+                        ++acc;
+                    }
+                }
+                if (a.acclog) a.acclog[ch.log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
+            }
+            __syncthreads();
+        }
+
+        // box-length trial (fav, src/jmmMCState.cpp:2161-2293)
+        VolProp vp{};
+        uint32_t bad = 0;
+        double vpart = 0.0;
+        if (npt) {
+            vp = vol_propose(ch, L, sweep);
+            if (vp.valid && act)
+                vpart = scaled_delta<Real, KIND>(q, gmin, X, N, per, L, vp.Lp, vp.sc, w, start,
+                                                 start + size, 1, bad);
+        }
+        {
+            double v[2] = {dUs, vpart};
+            uint32_t k2[2] = {acc, bad};
+            seg_allreduce<T, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            U = U + v[0];
+            vpart = v[1];
+            acc = k2[0];
+            bad = k2[1];
+        }
+        acc_total += acc;
+        uint32_t accv = 0;
+        if (npt) {
+            ++n_vt;
+            if (vp.valid && !bad && vol_accept(vpart, vp.dL, vp.lns, ch.P, beta, vp.jac, vp.u)) {
+                for (uint32_t p = start; p < start + size; ++p) X[p] = X[p] * vp.sc;
+                L = vp.Lp;
+                U = U + vpart;
+                accv = 1;
+                ++n_av;
+            }
+            __syncthreads();
+        }
+
+        // drift guard (ECheck, src/jmmMCState.cpp:1965-2095), always FP64
+        const int need = act && ch.eci && ((sweep + 1) % ch.eci == 0);
+        if (__syncthreads_or(need)) {
+            double v[1] = {need ? energy_part<KIND>(q, X, N, per, L, w, start, start + size, 1)
+                                : 0.0};
+            uint32_t k1[1] = {0};
+            seg_allreduce<T, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            if (need) {
+                const double dr = fabs(v[0] - U);
+                if (dr > mdrift) mdrift = dr;
+                U = v[0];
+                ++n_rc;
+            }
+        }
+
+        if (leader) {
+            const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
+            a.sU[row] = U;
+            a.sL[row] = L;
+            a.sA[row] = make_uint2(acc, accv);
+        }
+    }
+
+    {
+        double vz[1] = {0.0};
+        uint32_t k4[4] = {n_tr, n_wall, n_core, n_terms};
+        seg_allreduce<T, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        if (leader) {
+            DevState &o = a.states[ic];
+            o.L = L;
+            o.U = U;
+            o.sweep = sweep;
+            o.cnt[0] += k4[0];
+            o.cnt[1] += acc_total;
+            o.cnt[2] += k4[1];
+            o.cnt[3] += k4[2];
+            o.cnt[4] += k4[3];
+            o.cnt[5] += n_vt;
+            o.cnt[6] += n_av;
+            o.cnt[7] += n_rc;
+            o.max_drift = mdrift;
+        }
+    }
+    __syncthreads();
+    if (!GX) {
+        double *xg = a.x + bd.x_off;
+        for (uint32_t k = tid; k < bd.lds_doubles; k += T) xg[k] = smem[k];
+    }
+}
+
+// Persistent kernel: every workgroup pulls bundles from the queue until it is
+// empty; all waves of a workgroup leave together when the queue is drained.
+template <typename Real, int T, bool GX>
+__global__ void __launch_bounds__(T) k_sweep_batched(SweepArgs a)
+{
+    extern __shared__ double smem[];
+    __shared__ BatchedScratch<T> sc;
+    const uint32_t soff = a.ctrl->sample_off;
+    for (;;) {
+        if (threadIdx.x == 0) sc.item = atomicAdd(&a.ctrl->queue, 1u);
+        __syncthreads();
+        const uint32_t item = sc.item;
+        __syncthreads();
+        if (item >= a.n_items) break;
+        const DevBundle bd = a.bundles[item];
+        switch (bd.kind) {
+        case POT_LJ: run_bundle<Real, POT_LJ, T, GX>(a, bd, smem, sc, soff); break;
+        case POT_HARMONIC: run_bundle<Real, POT_HARMONIC, T, GX>(a, bd, smem, sc, soff); break;
+        case POT_HRSW: run_bundle<Real, POT_HRSW, T, GX>(a, bd, smem, sc, soff); break;
+        default: run_bundle<Real, POT_YUKAWA, T, GX>(a, bd, smem, sc, soff); break;
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- plain form
+
+template <int KIND, bool GX>
+__device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_t soff)
+{
+    const uint32_t lane = threadIdx.x;
+    const DevChain &ch = a.chains[ic];
+    const Pot q = ch.pot;
+    const int64_t N = ch.N;
+    const uint32_t w = ch.w, B = ch.B, smax = ch.smax;
+    const bool per = (ch.flags & F_PERIODIC) != 0;
+    const bool npt = (ch.flags & F_NPT) != 0;
+    const double beta = ch.beta, gmin = ch.gmin, max_step = ch.max_step;
+    const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+    double *X;
+    if (GX) {
+        X = a.x + ch.x_off;
+    } else {
+        for (uint32_t k = lane; k < (uint32_t)N; k += 64) smem[k] = a.x[ch.x_off + k];
+        X = smem;
+    }
+    __syncthreads();
+    const DevState st = a.states[ic];
+    double L = st.L, U = st.U, mdrift = st.max_drift;
+    uint64_t sweep = st.sweep;
+    const uint32_t qb = (uint32_t)N / B, rem = (uint32_t)N % B;
+    const uint32_t side = lane >> 5, kk = (lane & 31) + 1;
+    uint64_t n_tr = 0, n_wall = 0, n_core = 0, n_terms = 0, acc_total = 0;
+    uint32_t n_vt = 0, n_av = 0, n_rc = 0;
+
+    for (uint32_t sw = 0; sw < a.n_sweeps; ++sw, ++sweep) {
+        double dUs = 0.0;
+        uint32_t acc = 0;
+        for (uint32_t t = 0; t < smax; ++t) {
+            for (uint32_t b = 0; b < B; ++b) {
+                const uint32_t size = qb + (b < rem ? 1u : 0u);
+                if (t >= size) continue;
+                const uint32_t i = b * qb + (b < rem ? b : rem) + t;
+                uint32_t rr[4];
+                philox4x32_10(i, (uint32_t)sweep, ch.chain_id, 0u, key0, key1, rr);
+                const double a1 = u53(rr[0], rr[1]);
+                const double u = u53(rr[2], rr[3]);
+                const double md = ((a1 - 0.5) * 2.0) * max_step;
+                const double xi = X[i];
+                const double xn = xi + md;
+                ++n_tr;
+                bool ok = true;
+                if (!per && fabs(xn) > L / 2.0) {
+                    ok = false;
+                    ++n_wall;
+                } else if (N >= 2) {
+                    double xl, xr;
+                    if ((nbpos(X, (int64_t)i - 1, N, per, L, xl) && (xn - xl) < gmin) ||
+                        (nbpos(X, (int64_t)i + 1, N, per, L, xr) && (xr - xn) < gmin)) {
+                        ok = false;
+                        ++n_core;
+                    }
+                }
+                bool accd = false;
+                if (ok) {
+                    double term = 0.0;
+                    uint32_t tm = 0;
+                    for (uint32_t base = 0; base < w; base += 32) {
+                        const uint32_t k = base + kk;
+                        if (k <= w) {
+                            const int64_t j = side ? (int64_t)i + k : (int64_t)i - k;
+                            double xj;
+                            if (nbpos(X, j, N, per, L, xj)) {
+                                const double dold = side ? xj - xi : xi - xj;
+                                const double dnew = side ? xj - xn : xn - xj;
+                                if (!(dold > q.rc && dnew > q.rc)) {
+                                    term += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
+                                    tm += 2;
+                                }
+                            }
+                        }
+                    }
+                    const double dE = wave_sum(term);
+                    n_terms += wave_sum(tm);
+                    accd = dE <= 0.0 || exp(-dE * beta) > u;
+                    if (accd) {
+                        if (lane == 0) X[i] = xn;
+                        dUs += dE;
+                        ++acc;
+                    }
+                    __syncthreads();
+                }
+                if (a.acclog && lane == 0)
+                    a.acclog[ch.log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
+            }
+        }
+        U = U + dUs;
+        acc_total += acc;
+
+        uint32_t accv = 0;
+        if (npt) {
+            ++n_vt;
+            const VolProp vp = vol_propose(ch, L, sweep);
+            if (vp.valid) {
+                uint32_t bad = 0;
+                double part = scaled_delta<double, KIND>(q, gmin, X, N, per, L, vp.Lp, vp.sc, w,
+                                                         lane, (uint32_t)N, 64, bad);
+                part = wave_sum(part);
+                bad = wave_sum(bad);
+                if (!bad && vol_accept(part, vp.dL, vp.lns, ch.P, beta, vp.jac, vp.u)) {
+                    __syncthreads();
+                    for (uint32_t p = lane; p < (uint32_t)N; p += 64) X[p] = X[p] * vp.sc;
+                    L = vp.Lp;
+                    U = U + part;
+                    accv = 1;
+                    ++n_av;
+                }
+                __syncthreads();
+            }
+        }
+        if (ch.eci && ((sweep + 1) % ch.eci == 0)) {
+            const double Uf = wave_sum(energy_part<KIND>(q, X, N, per, L, w, lane, (uint32_t)N, 64));
+            const double dr = fabs(Uf - U);
+            if (dr > mdrift) mdrift = dr;
+            U = Uf;
+            ++n_rc;
+        }
+        if (lane == 0) {
+            const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
+            a.sU[row] = U;
+            a.sL[row] = L;
+            a.sA[row] = make_uint2(acc, accv);
+        }
+    }
+    if (lane == 0) {
+        DevState &o = a.states[ic];
+        o.L = L;
+        o.U = U;
+        o.sweep = sweep;
+        o.cnt[0] += n_tr;
+        o.cnt[1] += acc_total;
+        o.cnt[2] += n_wall;
+        o.cnt[3] += n_core;
+        o.cnt[4] += n_terms;
+        o.cnt[5] += n_vt;
+        o.cnt[6] += n_av;
+        o.cnt[7] += n_rc;
+        o.max_drift = mdrift;
+    }
+    __syncthreads();
+    if (!GX)
+        for (uint32_t k = lane; k < (uint32_t)N; k += 64) a.x[ch.x_off + k] = smem[k];
+}
+
+template <bool GX>
+__global__ void __launch_bounds__(64) k_sweep_plain(SweepArgs a)
+{
+    extern __shared__ double smem[];
+    const uint32_t soff = a.ctrl->sample_off;
+    const uint32_t ic = a.plain_list[blockIdx.x];
+    switch (a.chains[ic].kind) {
+    case POT_LJ: run_plain<POT_LJ, GX>(a, ic, smem, soff); break;
+    case POT_HARMONIC: run_plain<POT_HARMONIC, GX>(a, ic, smem, soff); break;
+    case POT_HRSW: run_plain<POT_HRSW, GX>(a, ic, smem, soff); break;
+    default: run_plain<POT_YUKAWA, GX>(a, ic, smem, soff); break;
+    }
+}
+
+__global__ void k_begin(Ctrl *c, uint32_t n)
+{
+    c->queue = 0;
+    c->sample_off = c->next_off;
+    c->next_off = c->next_off + n;
+}
+
+// ------------------------------------------------------- differentiation
+
+// One workgroup per configuration.  Pair arithmetic in Real, sums in FP64.
+// obs = (U, W, dU/dtheta0, dU/dtheta1, dW/dtheta0, dW/dtheta1); grad = dU/dx.
+template <typename Real, int KIND>
+__device__ void run_obs(const DevChain &ch, const double *X, double L, double *obs, double *grad)
+{
+    __shared__ double red[4 * 6];
+    const uint32_t tid = threadIdx.x;
+    const Pot q = ch.pot;
+    const int64_t N = ch.N;
+    const uint32_t w = ch.w;
+    const bool per = (ch.flags & F_PERIODIC) != 0;
+    const Real sh[3] = {Real(ch.sh[0]), Real(ch.sh[1]), Real(ch.sh[2])};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t i = tid; i < (uint32_t)N; i += 256) {
+        const double xi = X[i];
+        double g = 0.0;
+        for (uint32_t k = 1; k <= w; ++k) {
+            double xj;
+            if (!nbpos(X, (int64_t)i + k, N, per, L, xj)) break;
+            const double d = xj - xi;
+            if (d > q.rc) break;
+            if (!inside64<KIND>(q, d)) continue;
+            Real o[6];
+            pair_obs<KIND, Real>(q, Real(d), sh, o);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) acc[m] += (double)o[m];
+            g += (double)o[1] / d;
+        }
+        if (grad) {
+            for (uint32_t k = 1; k <= w; ++k) {
+                double xj;
+                if (!nbpos(X, (int64_t)i - k, N, per, L, xj)) break;
+                const double d = xi - xj;
+                if (d > q.rc) break;
+                if (!inside64<KIND>(q, d)) continue;
+                Real o[6];
+                pair_obs<KIND, Real>(q, Real(d), sh, o);
+                g -= (double)o[1] / d;
+            }
+            grad[i] = g;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[m] = wave_sum(acc[m]);
+    if ((tid & 63) == 0)
+        for (int m = 0; m < 6; ++m) red[(tid >> 6) * 6 + m] = acc[m];
+    __syncthreads();
+    if (tid == 0)
+        for (int m = 0; m < 6; ++m) obs[m] = ((red[m] + red[6 + m]) + red[12 + m]) + red[18 + m];
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) k_observables(ObsArgs a)
+{
+    const uint32_t c = blockIdx.x;
+    const DevChain &ch = a.standalone ? a.one : a.chains[a.list[c]];
+    const double *X;
+    double L, *grad;
+    double *obs;
+    if (a.standalone) {
+        X = a.x + (uint64_t)c * ch.N;
+        L = a.L[c];
+        grad = a.grad ? a.grad + (uint64_t)c * ch.N : nullptr;
+        obs = a.obs + (uint64_t)c * 6;
+    } else {
+        const uint32_t ic = a.list[c];
+        X = a.x + ch.x_off;
+        L = a.L[ic];
+        grad = a.grad ? a.grad + ch.x_off : nullptr;
+        obs = a.obs + (uint64_t)ic * 6;
+    }
+    switch (ch.kind) {
+    case POT_LJ: run_obs<Real, POT_LJ>(ch, X, L, obs, grad); break;
+    case POT_HARMONIC: run_obs<Real, POT_HARMONIC>(ch, X, L, obs, grad); break;
+    case POT_HRSW: run_obs<Real, POT_HRSW>(ch, X, L, obs, grad); break;
+    default: run_obs<Real, POT_YUKAWA>(ch, X, L, obs, grad); break;
+    }
+}
+
+// Copies every chain's current length out of the state array (observables
+// of engine chains).
+__global__ void k_lengths(const DevState *st, double *L, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) L[i] = st[i].L;
+}
+
+// ------------------------------------------------------------ launchers
+
+hipError_t launch_begin(Ctrl *c, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, s, c, n);
+    return hipGetLastError();
+}
+
+template <typename Real, int T, bool GX>
+static hipError_t launch_batched_t(const SweepArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_sweep_batched<Real, T, GX>), dim3(grid), dim3(T), lds, s, a);
+    return hipGetLastError();
+}
+
+const void *batched_kernel(int fp32, int T, int gx)
+{
+    if (fp32) {
+        if (T == 1024) return gx ? (const void *)k_sweep_batched<float, 1024, true>
+                                 : (const void *)k_sweep_batched<float, 1024, false>;
+        return gx ? (const void *)k_sweep_batched<float, 256, true>
+                  : (const void *)k_sweep_batched<float, 256, false>;
+    }
+    if (T == 1024) return gx ? (const void *)k_sweep_batched<double, 1024, true>
+                             : (const void *)k_sweep_batched<double, 1024, false>;
+    return gx ? (const void *)k_sweep_batched<double, 256, true>
+              : (const void *)k_sweep_batched<double, 256, false>;
+}
+
+hipError_t launch_batched(const SweepArgs &a, int fp32, int T, int gx, uint32_t grid, size_t lds,
+                          hipStream_t s)
+{
+    if (fp32) {
+        if (T == 1024) return gx ? launch_batched_t<float, 1024, true>(a, grid, lds, s)
+                                 : launch_batched_t<float, 1024, false>(a, grid, lds, s);
+        return gx ? launch_batched_t<float, 256, true>(a, grid, lds, s)
+                  : launch_batched_t<float, 256, false>(a, grid, lds, s);
+    }
+    if (T == 1024) return gx ? launch_batched_t<double, 1024, true>(a, grid, lds, s)
+                             : launch_batched_t<double, 1024, false>(a, grid, lds, s);
+    return gx ? launch_batched_t<double, 256, true>(a, grid, lds, s)
+              : launch_batched_t<double, 256, false>(a, grid, lds, s);
+}
+
+const void *plain_kernel(int gx)
+{
+    return gx ? (const void *)k_sweep_plain<true> : (const void *)k_sweep_plain<false>;
+}
+
+hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, hipStream_t s)
+{
+    if (gx) hipLaunchKernelGGL(k_sweep_plain<true>, dim3(grid), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL(k_sweep_plain<false>, dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s)
+{
+    if (fp32) hipLaunchKernelGGL(k_observables<float>, dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_observables<double>, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lengths, dim3((n + 255) / 256), dim3(256), 0, s, st, L, n);
+    return hipGetLastError();
+}
+
+}  // namespace jmm

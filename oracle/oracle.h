@@ -1,0 +1,139 @@
+/*
+ * oracle.h -- CPU restatement of jmmOneDMC's Metropolis chain loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  Nothing in the product links it, and the product has no CPU
+ * fallback that routes through it.
+ *
+ * What it restates (reference file:line, relative to the reference repo):
+ *   pair potentials          src/pot.cpp:19-141 (phiLJcut, phiLJinfcutoff,
+ *                            phiHarmoniccut) + the hard-rod/square-well and
+ *                            Yukawa potentials the configurations add
+ *   displacement trial       src/jmmMCState.cpp:1160-1464 (qad2)
+ *   volume trial             src/jmmMCState.cpp:2161-2293 (fav),
+ *                            src/jmmMCState.cpp:1648-1730 (qavLJ)
+ *   full energy              src/jmmMCState.cpp:853-1003 (fad),
+ *                            src/jmmMCState.cpp:1965-2095 (ECheck)
+ *   lattice start            src/jmmMCState.cpp:559-562
+ *   thermo samples           src/jmmMCState.cpp:1941-1961 (updateThermo)
+ *
+ * The sweep order, the Philox keying and the hard-core rule are the engine's
+ * (they replace the reference's with-replacement trial selection and GSL
+ * taus2 stream); they are restated here so the oracle replays the engine move
+ * for move.
+ *
+ * Parity pinning, per function (tests/test_oracle_*.py):
+ *   orc_philox4x32_10  Random123 known-answer vectors + rocRAND's
+ *                      philox4x32_10 engine on the host
+ *   orc_phi            closed forms (LJ zero at sigma, minimum -eps at
+ *                      2^(1/6) sigma with zero virial, value at the cutoff,
+ *                      inclusive/strict cutoffs, harmonic penalty) and
+ *                      central finite differences for the virial/gradients
+ *   orc_trial          brute-force O(N^2) recompute before/after on tiny N;
+ *                      hand-chosen (x, md, u) accept triples
+ *   orc_full_energy    brute-force all-pairs sum (orc_brute_energy, which has
+ *                      no window logic) on random and lattice inputs
+ *   orc_volume_trial   hand-chosen (L, dl, dU, u) acceptance cases and the
+ *                      LJ scaling identity for an infinite cutoff
+ *   orc_sweep          exact 1D NPT averages (Takahashi gas of independent
+ *                      gaps, integrated by quadrature) within 3 standard
+ *                      errors; incremental energy == full recompute
+ *   orc_observables    central finite differences in theta and x
+ *   orc_det_exp        libm exp to 2 ulp over the move range
+ */
+#ifndef JMM_ORACLE_H
+#define JMM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_POT_LJ = 0, ORC_POT_HARMONIC = 1, ORC_POT_HRSW = 2, ORC_POT_YUKAWA = 3 };
+enum { ORC_WALLS = 0, ORC_PERIODIC = 1 };
+enum { ORC_NLT = 0, ORC_NPT = 1 };
+enum { ORC_VOL_LINEAR = 0, ORC_VOL_LOG = 1 };
+
+/* One chain's configuration (the hot-path fields of the reference's MCInput,
+ * src/jmmMCState.h:66-101, plus the engine's additive fields). */
+typedef struct orc_spec {
+    int32_t potential, boundary, ensemble, vol_move;
+    int32_t nbn;       /* -1: no neighbour-number limit (reference NBN) */
+    int32_t shifted;   /* 1: subtract phi(rc) inside the cutoff */
+    uint64_t N;
+    double T, P, L0;   /* L0: initial box length */
+    double max_step;   /* displacement half-width (reference MAXSTEP) */
+    double max_dv;     /* volume half-width: dL (linear) or d ln L (log) */
+    double cutoff;     /* INFINITY allowed for walls */
+    double pot_p[4];
+    uint64_t seed;
+    uint32_t chain_id; /* Philox stream id */
+    uint32_t eci;      /* full-energy recompute interval in sweeps, 0 = never */
+    uint32_t bcap;     /* maximum number of blocks (power of two) */
+    uint32_t pad;
+} orc_spec;
+
+typedef struct orc_sample {
+    double U, L;
+    uint32_t acc_d, acc_v;
+} orc_sample;
+
+typedef struct orc_chain orc_chain;
+
+/* RNG */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_u53(uint32_t a, uint32_t b);
+double orc_det_exp(double x);
+
+/* potentials: out[0]=E, out[1]=W=-d*dphi/dd, out[2]=dE/dtheta0, out[3]=dE/dtheta1,
+ * out[4]=dW/dtheta0, out[5]=dW/dtheta1, out[6]=HV, out[7]=1 if d is inside the
+ * hard core (infinite energy), out[8]=1 if inside the cutoff. */
+void orc_phi(const orc_spec *s, double d, double L, double out[9]);
+/* The reference's own 9-component LJ output layout (src/pot.cpp:90-100). */
+void orc_phi_ljcut_ref(double d, double cutoff, int virial, double out[9]);
+void orc_phi_harmonic_ref(double d, double cutoff, double L, double out[2]);
+
+/* schedule */
+void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w);
+
+/* chain lifecycle */
+orc_chain *orc_chain_new(const orc_spec *s);
+void orc_chain_free(orc_chain *c);
+void orc_chain_get(const orc_chain *c, double *x, double *L, double *U, uint64_t *sweep);
+void orc_chain_set(orc_chain *c, const double *x, double L, uint64_t sweep);
+uint64_t orc_chain_counter(const orc_chain *c, int which);
+
+/* moves */
+int orc_trial(orc_chain *c, uint64_t i, uint64_t sweep, double *dE_out);
+int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out);
+int orc_volume_trial(orc_chain *c, uint64_t sweep, double *dU_out);
+int orc_volume_trial_with(orc_chain *c, double dv, double u, double *dU_out);
+int orc_accept(double dE, double beta, double u);
+int orc_vol_accept(double dU, double dL, double ln_s, double P, double beta,
+                   double jac, double u);
+
+/* sweeps: writes n samples; accept_log (nullable) receives one byte per
+ * displacement trial, indexed [sweep][particle], n*N bytes */
+void orc_sweeps(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *accept_log);
+/* the reference's cost model: every trial sums over all N-1 partners
+ * (src/jmmMCState.cpp:1213-1348); same chain, same results up to rounding */
+void orc_sweeps_allpairs(orc_chain *c, uint64_t n, orc_sample *samples);
+
+/* energies */
+double orc_full_energy(const orc_chain *c);
+double orc_brute_energy(const orc_chain *c);
+double orc_full_energy_x(const orc_spec *s, const double *x, double L);
+double orc_brute_energy_x(const orc_spec *s, const double *x, double L);
+/* U, W, dU/dtheta[2], dW/dtheta[2] into obs[6]; dU/dx into grad (nullable) */
+void orc_observables_x(const orc_spec *s, const double *x, double L, double *obs, double *grad);
+
+/* CPU baseline: run n_sweeps on every chain, one chain per thread */
+double orc_run_batch(const orc_spec *specs, int n_chains, uint64_t n_sweeps, int n_threads,
+                     int allpairs, double *U_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

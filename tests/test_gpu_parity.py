@@ -1,0 +1,87 @@
+"""GPU parity: the HIP kernels against the CPU oracle on the same seeded
+inputs (same Philox stream, same block schedule).
+
+FP64 forms: accept/reject decisions equal move for move, positions equal bit
+for bit, energies within 1e-10 relative.  FP32 form: decisions equal on these
+sizes (a flip needs a near-threshold tie, |ln u + beta dE| below the FP32
+error of dE, about 1e-7 per trial), energies within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from jmmonedmc_amd.engine import Engine
+
+from helpers import CASES, oracle_spec, rel
+
+SWEEPS = 40
+
+
+def run_pair(oracle, chains, form, sweeps=SWEEPS):
+    print(f"[gpu] form={form} chains={len(chains)} N={[c.N for c in chains][:6]} sweeps={sweeps}",
+          flush=True)
+    eng = Engine(chains, form=form, log_accepts=True, sample_capacity=sweeps, block_sweeps=16)
+    eng.run(sweeps)
+    logs = eng.read_accept_log(sweeps)
+    smp = eng.read_samples(sweeps)
+    out = []
+    for c, ch in enumerate(chains):
+        orc = oracle.Chain(oracle_spec(oracle, ch))
+        osmp, olog = orc.sweeps(sweeps, log=True)
+        out.append((eng.state(c), orc.state(), logs[c].ravel(), olog, smp[c], osmp,
+                    eng.counters(c), orc))
+    return eng, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fp64_move_for_move(gpu, oracle, form, case):
+    chains = CASES[case]()
+    eng, res = run_pair(oracle, chains, form)
+    for c, ((x, L, U, sw), (ox, oL, oU, osw), lg, olg, smp, osmp, cnt, orc) in enumerate(res):
+        assert sw == osw == SWEEPS
+        mism = np.flatnonzero(lg != olg)
+        assert mism.size == 0, f"chain {c}: first decision mismatch at trial {mism[:5]}"
+        assert np.array_equal(x, ox), f"chain {c}: positions differ"
+        assert L == oL
+        assert rel(U, oU) <= 1e-10
+        np.testing.assert_array_equal(smp["acc_d"], osmp["acc_d"])
+        np.testing.assert_array_equal(smp["acc_v"], osmp["acc_v"])
+        np.testing.assert_array_equal(smp["L"], osmp["L"])
+        assert np.all(np.abs(smp["U"] - osmp["U"]) <= 1e-10 * np.maximum(1, np.abs(osmp["U"])))
+        assert cnt["trials"] == orc.counter("trials")
+        assert cnt["acc_d"] == orc.counter("acc_d")
+        assert cnt["core_rejects"] == orc.counter("hc_rej")
+        assert cnt["wall_rejects"] == orc.counter("wall_rej")
+        assert cnt["pair_terms"] == orc.counter("pair_terms")
+    eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fp32_decisions_and_energy(gpu, oracle, case):
+    chains = CASES[case]()
+    eng, res = run_pair(oracle, chains, "batched_fp32")
+    for c, ((x, L, U, sw), (ox, oL, oU, osw), lg, olg, smp, osmp, cnt, orc) in enumerate(res):
+        mism = np.flatnonzero(lg != olg)
+        assert mism.size == 0, f"chain {c}: FP32 decision flips at {mism[:5]}"
+        assert np.array_equal(x, ox)
+        assert rel(U, oU) <= 1e-5
+    eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64", "batched_fp32"])
+def test_full_energy_matches_brute_force(gpu, oracle, form):
+    chains = CASES["mixed_batch"]()
+    eng = Engine(chains, form=form, sample_capacity=64)
+    eng.run(30)
+    eng.sync()
+    Uf = eng.full_energy()
+    tol_running = 1e-10 if form != "batched_fp32" else 1e-5
+    for c, ch in enumerate(chains):
+        x, L, U, _ = eng.state(c)
+        ub = oracle.brute_energy_x(oracle_spec(oracle, ch), x, L)
+        assert rel(Uf[c], ub) <= 1e-12
+        assert rel(U, ub) <= tol_running
+    eng.close()
