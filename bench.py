@@ -1,0 +1,234 @@
+"""Benchmark: trial moves/s of the batched FP32 engine on one MI355X, beside
+the plain FP64 GPU form and the FP64 CPU oracle, on the 4096-chain mixed
+workload (C3).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+
+A step is one sweep block (--block-sweeps sweeps, default 100) of every chain;
+the default K=100 steps is the workload's full 1e4 sweeps.  Prints one JSON
+line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "trial moves/s on 1 MI355X, batched FP32 vs plain FP64, at ≤3σ ⟨U⟩ agreement"
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--block-sweeps", type=int, default=100)
+    p.add_argument("--workload", default="c3", choices=["c3", "c2", "c5"])
+    p.add_argument("--plain-steps", type=int, default=1, help="sweep blocks timed for the plain FP64 form")
+    p.add_argument("--forms", default="fp32,fp64,plain",
+                   help="comma list of forms to time: fp32 (batched FP32, the value), fp64 "
+                        "(batched FP64), plain (plain FP64)")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-sweeps", type=int, default=0, help="CPU baseline sweeps (0 = auto ~15 s)")
+    p.add_argument("--graph", type=int, default=1)
+    return p.parse_args()
+
+
+def workload(name):
+    from jmmonedmc_amd import configs
+    if name == "c3":
+        return configs.c3(), ("C3: 4096 chains, N log-uniform [32, 8192], T~U[0.5,2], P~U[0.1,2], "
+                              "1/2 shifted LJ rc=3 + 1/2 hard rod 1 + square well 1.5, NPT ln L "
+                              "+-0.01, step +-0.5, density 0.5 start, FP64 recompute every 100 sweeps")
+    if name == "c2":
+        return configs.c2(), "C2: 256 chains, N=4096, NPT T=1 P=0.5, shifted LJ rc=3"
+    return configs.c5(), "C5: 64 chains, N=65536, NPT T=1.5 P=1, Yukawa rc=20 + hard core 0.5"
+
+
+def algorithmic_flops(eng, chains, before):
+    """sum over chains of f_pair*(n_old+n_new) + (n_old+n_new) + 8 per trial,
+    with n_old+n_new the pair evaluations the kernel counted."""
+    from jmmonedmc_amd.configs import flop_per_pair
+    tot = 0.0
+    terms_tot = 0
+    trials_tot = 0
+    for c, ch in enumerate(chains):
+        cnt = eng.counters(c)
+        terms = cnt["pair_terms"] - before[c]["pair_terms"]
+        trials = cnt["trials"] - before[c]["trials"]
+        tot += flop_per_pair(ch.potential) * terms + terms + 8.0 * trials
+        terms_tot += terms
+        trials_tot += trials
+    return tot, terms_tot, trials_tot
+
+
+def block_sem(x, nblocks=20):
+    n = len(x) // nblocks * nblocks
+    if n < nblocks:
+        return float("nan")
+    m = x[:n].reshape(nblocks, -1).mean(axis=1)
+    return float(m.std() / math.sqrt(nblocks))
+
+
+def run_form(chains, form, steps, warmup, bs, graph, label):
+    from jmmonedmc_amd.engine import Engine
+    cap = max(steps, warmup, 1) * bs
+    eng = Engine(chains, form=form, sample_capacity=cap, block_sweeps=bs, use_graph=bool(graph))
+    for _ in range(warmup):
+        eng.run(bs)
+    eng.sync()
+    eng.read_samples(cap)
+    eng.kernel_time(reset=True)
+    before = [eng.counters(c) for c in range(len(chains))]
+    print(f"[bench] {label}: timing {steps} blocks of {bs} sweeps", file=sys.stderr, flush=True)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.run(bs)
+    eng.sync()
+    t1 = time.perf_counter()
+    launches, ms, sweeps = eng.kernel_time(reset=True)
+    flops, terms, trials = algorithmic_flops(eng, chains, before)
+    smp = eng.read_samples(cap)
+    eng.close()
+    wall = t1 - t0
+    return dict(wall_s=wall, trials=trials, terms=terms, flops=flops, launches=launches,
+                kernel_ms=ms, sweeps=sweeps, samples=smp,
+                trials_per_s=trials / wall, ms_per_step=1e3 * wall / steps)
+
+
+def cpu_baseline(chains, sweeps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    specs = [pyoracle.make_spec(**ch.as_dict()) for ch in chains]
+    dt, _ = pyoracle.run_batch(specs, sweeps, threads)
+    trials = sweeps * sum(ch.N for ch in chains)
+    return trials / dt, dt, trials
+
+
+def calibrate_copy(mib):
+    """A device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    readings scale the sweep kernel's counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+    return 0
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this "
+                                   "build (chains shard across GPUs by design, see DESIGN.md)",
+                          "n_gpus": a.gpus}))
+        return 2
+    if a.calibrate_copy:
+        return calibrate_copy(a.calibrate_copy)
+    forms = set(a.forms.split(","))
+    chains, desc = workload(a.workload)
+    n_trials_per_sweep = sum(ch.N for ch in chains)
+
+    fp32 = run_form(chains, "batched_fp32", a.steps, a.warmup, a.block_sweeps, a.graph, "batched_fp32")
+    res = {}
+    if "fp64" in forms:
+        res["batched_fp64"] = run_form(chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps,
+                                       a.graph, "batched_fp64")
+    if "plain" in forms and a.plain_steps > 0:
+        res["plain_fp64"] = run_form(chains, "plain_fp64", a.plain_steps, 0, a.block_sweeps, 0,
+                                     "plain_fp64")
+
+    # <U>/N agreement of the FP32 run against the FP64 batched run (same
+    # chains, same seeds, same length): per-chain difference of means in units
+    # of the combined block standard error, second half of the run.
+    accuracy = None
+    if "batched_fp64" in res:
+        z = []
+        U32, U64 = fp32["samples"]["U"], res["batched_fp64"]["samples"]["U"]
+        h = U32.shape[1] // 2
+        for c in range(len(chains)):
+            s1, s2 = block_sem(U32[c, h:]), block_sem(U64[c, h:])
+            se = math.hypot(s1, s2)
+            if se > 0:
+                z.append(abs(U32[c, h:].mean() - U64[c, h:].mean()) / se)
+        z = np.array(z)
+        accuracy = {"observable": "<U> per chain, second half, 20-block SEM",
+                    "chains_compared": int(z.size),
+                    "frac_within_3sigma": float((z <= 3).mean()),
+                    "expected_frac_within_3sigma": 0.9973,
+                    "max_z": float(z.max()) if z.size else None}
+
+    # roofline of the dominant kernel (the batched FP32 sweep kernel)
+    achieved = fp32["flops"] / (fp32["kernel_ms"] * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
+            "flops_per_launch": fp32["flops"] / max(1, fp32["launches"]),
+            "avg_launch_ms": fp32["kernel_ms"] / max(1, fp32["launches"]),
+            "pair_evals_per_trial": fp32["terms"] / max(1, fp32["trials"])}
+    traffic_file = os.path.join(ROOT, "profiles", f"{a.workload}_sweep_traffic.txt")
+    if os.path.exists(traffic_file):
+        for line in open(traffic_file):
+            if line.startswith("hbm_bytes_per_launch:"):
+                roof["traffic"] = float(line.split(":", 1)[1])
+
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        sw = a.cpu_sweeps or max(1, int(15.0 * 1.2e7 * threads / n_trials_per_sweep))
+        v, dt, tr = cpu_baseline(chains, sw, threads)
+        cpu = {"value": v, "unit": "trial moves/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle, all {len(chains)} chains x {sw} sweeps ({tr:.3g} trials, "
+                         f"{dt:.1f} s), one chain per thread"}
+
+    plain = res.get("plain_fp64")
+    out = {
+        "metric": METRIC,
+        "value": fp32["trials_per_s"],
+        "unit": "trial moves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": fp32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": desc, "n_chains": len(chains), "particles": n_trials_per_sweep,
+                   "sweeps_timed": a.steps * a.block_sweeps, "block_sweeps": a.block_sweeps,
+                   "form": "batched_fp32", "parallelism": "1 GPU, chains packed into workgroups"},
+        "chain_sweeps_per_s": len(chains) * a.steps * a.block_sweeps / fp32["wall_s"],
+        "plain_fp64": plain and {"value": plain["trials_per_s"], "sweeps_timed": plain["sweeps"],
+                                 "ms_per_block": plain["ms_per_step"]},
+        "speedup_vs_plain_fp64": plain and fp32["trials_per_s"] / plain["trials_per_s"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "accuracy_fp32_vs_fp64": accuracy,
+    }
+    if "batched_fp64" in res:
+        b = res["batched_fp64"]
+        out["batched_fp64"] = {"value": b["trials_per_s"],
+                               "achieved_tflops": b["flops"] / (b["kernel_ms"] * 1e-3) / 1e12,
+                               "peak": PEAK_TFLOPS["fp64"]}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

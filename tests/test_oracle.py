@@ -1,0 +1,379 @@
+"""Pins for the CPU oracle (oracle/oracle.c) from outside itself: published
+known-answer vectors, closed forms, finite differences, brute-force sums, and
+exact properties of the schedule.  A plausible slip in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails one of these.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import CASES, hrsw, lj, oracle_spec, rel, yukawa
+
+from jmmonedmc_amd import _native as nat
+from jmmonedmc_amd.engine import Chain
+
+
+# ------------------------------------------------------------------ Philox
+
+# Random123 kat_vectors (philox4x32, 10 rounds): ctr, key -> out
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", PHILOX_KAT)
+def test_philox_known_answers(oracle, ctr, key, out):
+    assert tuple(oracle.philox(ctr, key)) == out
+
+
+def test_u53_range(oracle):
+    assert oracle.u53(0, 0) == 0.0
+    assert oracle.u53(0xFFFFFFFF, 0xFFFFFFFF) == 1.0 - 2.0 ** -53
+    assert oracle.u53(0x80000000, 0) == 0.5
+    # monotone in the high word
+    v = [oracle.u53(a, 0) for a in range(0, 1 << 32, 1 << 27)]
+    assert all(x < y for x, y in zip(v, v[1:]))
+
+
+def test_det_exp_against_libm(oracle):
+    xs = np.concatenate([np.linspace(-0.02, 0.02, 2001), np.linspace(-700, 700, 2001),
+                         [0.0, 1e-300, -1e-300, 0.5 * math.log(2), -0.5 * math.log(2)]])
+    for x in xs:
+        e = oracle.det_exp(float(x))
+        ref = math.exp(float(x))
+        assert abs(e - ref) <= 2 * math.ulp(ref), (x, e, ref)
+
+
+# ------------------------------------------------------------ potentials
+
+def spec_of(**kw):
+    import pyoracle
+    ch = Chain(N=8, L0=100.0, **kw)
+    return pyoracle.make_spec(**ch.as_dict())
+
+
+def test_lj_closed_forms(oracle):
+    s = spec_of(**lj(shifted=0))
+    assert oracle.phi(s, 1.0)[0] == 0.0
+    rmin = 2 ** (1 / 6)
+    o = oracle.phi(s, rmin)
+    assert abs(o[0] + 1.0) < 1e-14
+    assert abs(o[1]) < 1e-13  # virial vanishes at the minimum
+    assert abs(oracle.phi(s, 3.0)[0] - 4 * (3.0 ** -12 - 3.0 ** -6)) < 1e-18
+    assert abs(oracle.phi(s, 3.0)[0] - (-5.4794417e-3)) < 1e-9
+    # inclusive cutoff (src/pot.cpp:53)
+    assert oracle.phi(s, 3.0)[0] != 0.0
+    assert oracle.phi(s, math.nextafter(3.0, 4.0))[0] == 0.0
+    # shifted form vanishes at the cutoff and differs by phi(rc) inside
+    ss = spec_of(**lj(shifted=1))
+    assert oracle.phi(ss, 3.0)[0] == 0.0
+    assert abs((oracle.phi(ss, 1.3)[0] - oracle.phi(s, 1.3)[0]) - 5.4794417e-3) < 1e-9
+    # eps, sigma scaling: phi(sigma d; eps, sigma) = eps phi(d; 1, 1)
+    sg = spec_of(**lj(shifted=0, cutoff=10.0, pot_p=(2.5, 1.7, 0, 0)))
+    for d in (1.1, 1.5, 2.2):
+        assert rel(oracle.phi(sg, 1.7 * d)[0], 2.5 * oracle.phi(s, d)[0]) < 1e-13
+
+
+def test_lj_matches_reference_layout(oracle):
+    """General LJ at eps = sigma = 1, truncated, reproduces the reference's
+    phiLJcut components bit for bit (E, W, HV)."""
+    import ctypes as C
+    s = spec_of(**lj(shifted=0))
+    for d in np.linspace(0.6, 3.2, 101):
+        ref = np.zeros(9)
+        oracle.lib().orc_phi_ljcut_ref(float(d), 3.0, 1, ref.ctypes.data_as(C.POINTER(C.c_double)))
+        o = oracle.phi(s, float(d))
+        assert o[0] == ref[0] and o[1] == ref[1] and o[6] == ref[6]
+
+
+def test_harmonic_closed_forms(oracle):
+    s = spec_of(potential=nat.POT_HARMONIC, cutoff=2.0, pot_p=(0, 0, 0, 0), shifted=0)
+    assert oracle.phi(s, 0.0)[0] == 10e10 and oracle.phi(s, -0.5)[0] == 10e10
+    assert oracle.phi(s, 1.0)[0] == 0.0
+    assert oracle.phi(s, 1.5)[0] == 0.25
+    assert oracle.phi(s, 2.0)[0] == 0.0            # strict cutoff (src/pot.cpp:120)
+    assert oracle.phi(s, math.nextafter(2.0, 0))[0] > 0.99
+
+
+def test_hrsw_and_yukawa_closed_forms(oracle):
+    s = spec_of(**hrsw())
+    assert math.isinf(oracle.phi(s, 0.999)[0])
+    assert oracle.phi(s, 1.0)[0] == -1.0
+    assert oracle.phi(s, 1.4999)[0] == -1.0
+    assert oracle.phi(s, 1.5)[0] == 0.0
+    y = spec_of(**yukawa())
+    assert math.isinf(oracle.phi(y, 0.49)[0])
+    for d in (0.5, 1.0, 7.3, 20.0):
+        assert rel(oracle.phi(y, d)[0], math.exp(-0.2 * d) / d) < 1e-15
+    assert oracle.phi(y, 20.000001)[0] == 0.0
+
+
+@pytest.mark.parametrize("kw,ds", [
+    (lj(shifted=1, cutoff=5.0, pot_p=(1.3, 0.9, 0, 0)), (0.8, 1.0, 1.4, 2.9)),
+    (yukawa(pot_p=(1.7, 0.35, 0.5, 0)), (0.7, 2.0, 9.0)),
+    (dict(potential=nat.POT_HARMONIC, cutoff=5.0, pot_p=(0, 0, 0, 0), shifted=0), (0.5, 1.3, 2.0)),
+])
+def test_virial_and_theta_gradients_by_finite_differences(oracle, kw, ds):
+    s = spec_of(**kw)
+    h = 1e-6
+    for d in ds:
+        o = oracle.phi(s, d)
+        dphi = (oracle.phi(s, d + h)[0] - oracle.phi(s, d - h)[0]) / (2 * h)
+        assert abs(o[1] - (-d * dphi)) <= 1e-6 * max(1, abs(o[1]))
+        if kw["potential"] == nat.POT_HARMONIC:
+            continue
+        for k in (0, 1):
+            p = list(kw["pot_p"])
+            up, dn = list(p), list(p)
+            up[k] += h
+            dn[k] -= h
+            su = spec_of(**{**kw, "pot_p": tuple(up)})
+            sd = spec_of(**{**kw, "pot_p": tuple(dn)})
+            fd_e = (oracle.phi(su, d)[0] - oracle.phi(sd, d)[0]) / (2 * h)
+            fd_w = (oracle.phi(su, d)[1] - oracle.phi(sd, d)[1]) / (2 * h)
+            assert abs(o[2 + k] - fd_e) <= 1e-6 * max(1, abs(fd_e)), (d, k)
+            assert abs(o[4 + k] - fd_w) <= 1e-6 * max(1, abs(fd_w)), (d, k)
+
+
+# -------------------------------------------------------------- schedule
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_schedule_invariants(oracle, name):
+    import pyoracle
+    for ch in CASES[name]():
+        s = oracle_spec(pyoracle, ch)
+        B, M, w = oracle.schedule(s)
+        assert B & (B - 1) == 0 and B >= 1
+        N = ch.N
+        if B > 1:
+            assert N // B >= M                      # every block at least M_min long
+        g = {nat.POT_LJ: 0.5 * ch.pot_p[1], nat.POT_HRSW: ch.pot_p[0],
+             nat.POT_YUKAWA: ch.pot_p[2]}.get(ch.potential, 0.0)
+        if ch.nbn >= 0:
+            assert w <= ch.nbn and M <= ch.nbn + 1
+        if g > 0 and math.isfinite(ch.cutoff) and ch.nbn < 0 and M <= N:
+            assert M * g > ch.cutoff + 2 * ch.max_step
+            assert w >= min(N - 1, math.floor((ch.cutoff + ch.max_step) / g))
+        # engine and oracle compute the same schedule
+        from jmmonedmc_amd.engine import schedule
+        assert schedule(ch) == (B, M, w)
+
+
+def _phase_members(N, B, t):
+    q, r = divmod(N, B)
+    out = []
+    for b in range(B):
+        size = q + (b < r)
+        start = b * q + min(b, r)
+        if t < size:
+            out.append(start + t)
+    return out
+
+
+@pytest.mark.parametrize("name", ["c1_periodic_nvt", "hrsw_npt", "yukawa_npt", "lj_inf_nbn2_walls",
+                                  "npt_periodic_log"])
+def test_phase_trials_commute(oracle, name):
+    """Trials of one phase run in forward and in reverse order give the same
+    state bit for bit: the independence the parallel schedule relies on."""
+    import pyoracle
+    ch = CASES[name]()[0]
+    s = oracle_spec(pyoracle, ch)
+    B, M, w = oracle.schedule(s)
+    a = oracle.Chain(s)
+    a.sweeps(5)
+    x0, L0, _, sw = a.state()
+    rng = np.random.default_rng(0)
+    for t in range((ch.N + B - 1) // B):
+        members = _phase_members(ch.N, B, t)
+        md = rng.uniform(-ch.max_step, ch.max_step, len(members))
+        u = rng.uniform(0, 1, len(members))
+        fwd = oracle.Chain(s)
+        fwd.set_state(x0, L0, sw)
+        rev = oracle.Chain(s)
+        rev.set_state(x0, L0, sw)
+        for k in range(len(members)):
+            fwd.trial_with(members[k], md[k], u[k])
+        for k in reversed(range(len(members))):
+            rev.trial_with(members[k], md[k], u[k])
+        assert np.array_equal(fwd.state()[0], rev.state()[0]), f"phase {t}"
+        x0 = fwd.state()[0]
+
+
+# ------------------------------------------------------------ energies
+
+def _random_config(rng, N, L, gmin, walls):
+    """Uniform points with every gap >= gmin (and inside the walls)."""
+    free = L - N * gmin - (gmin if not walls else 0.0)
+    g = np.sort(rng.uniform(0, free, N))
+    x = g + gmin * np.arange(N)
+    x = x - L / 2 + (gmin / 2 if walls else 0.0)
+    return x
+
+
+@pytest.mark.parametrize("kw,gmin", [(lj(), 0.5), (lj(shifted=0), 0.5), (hrsw(), 1.0),
+                                     (yukawa(cutoff=6.0), 0.5),
+                                     (dict(potential=nat.POT_HARMONIC, cutoff=1.8,
+                                           pot_p=(0, 0, 0, 0), shifted=0), 0.05)])
+@pytest.mark.parametrize("boundary", [nat.WALLS, nat.PERIODIC])
+@pytest.mark.parametrize("nbn", [-1, 1, 3])
+def test_window_energy_equals_brute_force(oracle, kw, gmin, boundary, nbn):
+    import pyoracle
+    rng = np.random.default_rng(1)
+    for N in (2, 3, 5, 17, 64):
+        L = max(N * (gmin + 1.0), 2 * (kw["cutoff"] + 0.5) + 1)
+        ch = Chain(N=N, L0=L, boundary=boundary, nbn=nbn, **kw)
+        s = oracle_spec(pyoracle, ch)
+        x = _random_config(rng, N, L, gmin, boundary == nat.WALLS)
+        ub = oracle.brute_energy_x(s, x, L)
+        uw = oracle.full_energy_x(s, x, L)
+        assert rel(uw, ub) <= 1e-13, (N, uw, ub)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_trial_energy_change_equals_brute_force(oracle, name):
+    """dE of single trials equals brute(after) - brute(before) on every
+    accepted trial, and rejected trials leave the state untouched."""
+    import pyoracle
+    ch = CASES[name]()[0]
+    s = oracle_spec(pyoracle, ch)
+    c = oracle.Chain(s)
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        i = int(rng.integers(ch.N))
+        md = float(rng.uniform(-ch.max_step, ch.max_step))
+        x0, L, U0, _ = c.state()
+        b0 = c.brute_energy()
+        acc, dE = c.trial_with(i, md, 0.0)   # u = 0: every finite dE is accepted
+        x1, _, U1, _ = c.state()
+        b1 = c.brute_energy()
+        if acc:
+            assert abs((b1 - b0) - dE) <= 1e-9 * max(1.0, abs(b0)), (i, md)
+            assert x1[i] == x0[i] + md
+        else:
+            assert np.array_equal(x0, x1)
+
+
+def test_accept_rule_hand_cases(oracle):
+    L = oracle.lib()
+    beta = 1.0 / 0.7
+    assert L.orc_accept(0.0, beta, 0.999999) == 1          # dE <= 0 always accepted
+    assert L.orc_accept(-3.0, beta, 0.5) == 1
+    dE = 0.4
+    bf = math.exp(-dE * beta)
+    assert L.orc_accept(dE, beta, bf) == 0                 # tie: strict '>' rejects
+    assert L.orc_accept(dE, beta, math.nextafter(bf, 0)) == 1
+    assert L.orc_accept(dE, beta, math.nextafter(bf, 1)) == 0
+    assert L.orc_accept(1e4, beta, 0.0) == 0               # underflowed factor never accepted
+    # box-length trials: accept when bf >= 1, else bf > u
+    assert L.orc_vol_accept(-1.0, 0.0, 0.0, 1.0, 1.0, 10.0, 0.999) == 1
+    assert L.orc_vol_accept(0.0, 0.5, 0.0, 1.0, 1.0, 10.0, math.exp(-0.5)) == 0
+    assert L.orc_vol_accept(0.0, 0.5, 0.0, 1.0, 1.0, 10.0, math.nextafter(math.exp(-0.5), 0)) == 1
+    # Jacobian sign: growing the box (ln s > 0) helps acceptance
+    assert L.orc_vol_accept(0.3, 0.0, 0.05, 0.0, 1.0, 10.0, 0.95) == 1
+
+
+def test_wall_and_core_rejections(oracle):
+    import pyoracle
+    ch = Chain(N=4, L0=8.0, boundary=nat.WALLS, **lj(shifted=0))
+    c = oracle.Chain(oracle_spec(pyoracle, ch))
+    x, L, _, _ = c.state()                 # lattice -3, -1, 1, 3
+    acc, _ = c.trial_with(3, 1.2, 0.0)     # 4.2 > L/2: wall
+    assert acc == 0 and c.counter("wall_rej") == 1
+    acc, _ = c.trial_with(1, 1.6, 0.0)     # gap to particle 2 becomes 0.4 < 0.5: core
+    assert acc == 0 and c.counter("hc_rej") == 1
+    assert np.array_equal(c.state()[0], x)
+
+
+def test_volume_trial_energy_and_scaling_identity(oracle):
+    """Accepted box-length trials land on the brute-force energy of the scaled
+    configuration; with no cutoff the LJ energy obeys
+    E(s x) = s^-12 E12 - s^-6 E6 (the identity qavLJ uses,
+    src/jmmMCState.cpp:1658-1664)."""
+    import ctypes as C
+    import pyoracle
+    ch = Chain(N=12, L0=30.0, boundary=nat.WALLS, ensemble=nat.NPT, vol_move=nat.VOL_LINEAR,
+               max_dv=2.0, **lj(cutoff=math.inf, shifted=0))
+    s = oracle_spec(pyoracle, ch)
+    c = oracle.Chain(s)
+    c.sweeps(10)
+    x, L, U, _ = c.state()
+    e12 = e6 = 0.0
+    for i in range(12):
+        for j in range(i + 1, 12):
+            ref = np.zeros(9)
+            oracle.lib().orc_phi_ljcut_ref(x[j] - x[i], math.inf, 1,
+                                           ref.ctypes.data_as(C.POINTER(C.c_double)))
+            e12 += ref[2]
+            e6 += ref[4]
+    for dv in (-1.5, 0.7, 1.9):
+        sc = (L + dv) / L
+        want = sc ** -12 * e12 - sc ** -6 * e6
+        got = oracle.brute_energy_x(s, x * sc, L + dv)
+        assert rel(got, want) <= 1e-12
+    acc, dU = c.volume_trial_with(0.7, 0.0)
+    assert acc == 1
+    x2, L2, U2, _ = c.state()
+    assert L2 == L + 0.7
+    assert rel(U2, c.brute_energy()) <= 1e-12
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_incremental_energy_equals_recompute(oracle, name):
+    import pyoracle
+    for ch in CASES[name]():
+        c = oracle.Chain(oracle_spec(pyoracle, ch))
+        c.sweeps(60)
+        assert rel(c.state()[2], c.brute_energy()) <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["c1_walls_truncated", "npt_walls_linear", "lj_inf_walls_npt",
+                                  "harmonic_nbn1_walls", "hrsw_npt", "npt_periodic_log"])
+def test_windowed_sweeps_equal_all_partner_sweeps(oracle, name):
+    """The reference's cost model (all N-1 partners per trial) and the window
+    form make the same decisions and reach the same energies."""
+    import pyoracle
+    ch = CASES[name]()[0]
+    a = oracle.Chain(oracle_spec(pyoracle, ch))
+    b = oracle.Chain(oracle_spec(pyoracle, ch))
+    sa = a.sweeps(30)
+    sb = b.sweeps_allpairs(30)
+    assert np.array_equal(sa["acc_d"], sb["acc_d"])
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert np.allclose(sa["U"], sb["U"], rtol=1e-10, atol=1e-10)
+
+
+def test_observables_by_finite_differences(oracle):
+    """dU/dx and dU/dtheta of a random LJ configuration against central
+    differences of the brute-force energy."""
+    import pyoracle
+    rng = np.random.default_rng(3)
+    for boundary in (nat.WALLS, nat.PERIODIC):
+        ch = Chain(N=20, L0=40.0, boundary=boundary, **lj(pot_p=(1.1, 0.95, 0, 0)))
+        s = oracle_spec(pyoracle, ch)
+        x = _random_config(rng, 20, 40.0, 0.8, boundary == nat.WALLS)
+        obs, g = oracle.observables_x(s, x, 40.0)
+        assert rel(obs[0], oracle.brute_energy_x(s, x, 40.0)) <= 1e-13
+        h = 1e-6
+        for i in range(20):
+            xp, xm = x.copy(), x.copy()
+            xp[i] += h
+            xm[i] -= h
+            fd = (oracle.brute_energy_x(s, xp, 40.0) - oracle.brute_energy_x(s, xm, 40.0)) / (2 * h)
+            assert abs(g[i] - fd) <= 1e-6 * max(1, abs(fd))
+        for k, dp in ((0, (h, 0)), (1, (0, h))):
+            sp = oracle_spec(pyoracle, Chain(N=20, L0=40.0, boundary=boundary,
+                                            **lj(pot_p=(1.1 + dp[0], 0.95 + dp[1], 0, 0))))
+            sm = oracle_spec(pyoracle, Chain(N=20, L0=40.0, boundary=boundary,
+                                            **lj(pot_p=(1.1 - dp[0], 0.95 - dp[1], 0, 0))))
+            fd = (oracle.brute_energy_x(sp, x, 40.0) - oracle.brute_energy_x(sm, x, 40.0)) / (2 * h)
+            assert abs(obs[2 + k] - fd) <= 1e-6 * max(1, abs(fd))
+        # virial: W = -dU/d ln(scale) at fixed box for a periodic system
+        if boundary == nat.PERIODIC:
+            e = 1e-6
+            up = oracle.brute_energy_x(s, x * (1 + e), 40.0 * (1 + e))
+            dn = oracle.brute_energy_x(s, x * (1 - e), 40.0 * (1 - e))
+            assert abs(obs[1] + (up - dn) / (2 * e)) <= 1e-6 * max(1, abs(obs[1]))
