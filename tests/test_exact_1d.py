@@ -67,15 +67,20 @@ CASES = {"hrsw_periodic": case_hrsw_periodic, "tonks_periodic": case_tonks_perio
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_oracle_matches_exact_npt(oracle, name):
+    """12 independent chains (distinct seeds) of 40000 sweeps; the spread of
+    the per-chain means gives the standard error."""
     import pyoracle
-    ch, (L_ex, U_ex) = CASES[name]()
-    c = oracle.Chain(oracle_spec(pyoracle, ch))
-    c.sweeps(2000)                      # equilibrate
-    smp = c.sweeps(200000)
-    Lm, Ls = block_mean_sem(smp["L"])
-    Um, Us = block_mean_sem(smp["U"])
-    assert abs(Lm - L_ex) <= 3 * Ls, (Lm, L_ex, Ls)
-    assert abs(Um - U_ex) <= 3 * Us, (Um, U_ex, Us)
+    Ls, Us = [], []
+    for k in range(12):
+        ch, (L_ex, U_ex) = CASES[name](seed=700 + k)
+        c = oracle.Chain(oracle_spec(pyoracle, ch))
+        c.sweeps(2000)                      # equilibrate
+        smp = c.sweeps(40000)
+        Ls.append(smp["L"].mean())
+        Us.append(smp["U"].mean())
+    for vals, ex, what in ((np.array(Ls), L_ex, "L"), (np.array(Us), U_ex, "U")):
+        m, se = vals.mean(), vals.std(ddof=1) / math.sqrt(len(vals))
+        assert abs(m - ex) <= 3 * se, (what, m, ex, se)
 
 
 @pytest.mark.gpu
