@@ -1,0 +1,441 @@
+// mcstate.cpp -- the jmmMCState-style facade over the engine's C ABI.
+//
+// Output files keep the reference's formats: config.dat.mcs frames
+// (src/jmmMCState.cpp:1009-1013) and the 13-column thermo.dat.mcs
+// (header :566-568, rows :1916-1918).  Thermo block means are over per-sweep
+// samples (the reference samples every trial); the virial and hypervirial
+// columns are evaluated where the queued sweeps are run (print and check
+// points) and held for the sweeps in between.
+#include "jmmMCState.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "jmm_engine.h"
+
+struct MCState {
+    MCInput inp;
+    jmm_engine *eng = nullptr;
+    std::vector<jmm_chain_spec> specs;
+    unsigned long N = 0, numTrialTypes = 0, numSteps = 0, sn = 0, lastStep = 0;
+    unsigned long pending = 0, cap = 4096, sweepsDone = 0;
+    double rhoSysA = 0, rhoSysSA = 0, lA = 0, lSA = 0, EA = 0, ESA = 0, lEA = 0;
+    double VirA = 0, VirSA = 0, EVirA = 0, HVA = 0, HVSA = 0;
+    unsigned long nSamples = 0;
+    double E = 0, l = 0, Vir = 0, HV = 0;
+    FILE *cf = nullptr, *tf = nullptr;
+    std::vector<jmm_sample> buf;
+};
+
+static void die_on(int rc, const char *what)
+{
+    if (rc) std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, jmm_last_error());
+}
+
+static jmm_chain_spec spec_from_input(const MCInput &inp, unsigned long k, bool &ok)
+{
+    jmm_chain_spec s{};
+    ok = true;
+    const std::string pot = inp.potStr;
+    s.shifted = 0;
+    s.cutoff = inp.potCutOff;
+    s.pot_p[0] = 1.0;
+    s.pot_p[1] = 1.0;
+    if (pot == "LJ") {
+        s.potential = JMM_POT_LJ;
+        s.cutoff = INFINITY;  // reference: LJ ignores the cutoff (src/jmmMCState.cpp:292-294)
+    } else if (pot == "LJcut") {
+        s.potential = JMM_POT_LJ;
+    } else if (pot == "LJshift") {
+        s.potential = JMM_POT_LJ;
+        s.shifted = 1;
+    } else if (pot == "HARMONIC") {
+        s.potential = JMM_POT_HARMONIC;
+    } else if (pot == "HRSW") {
+        s.potential = JMM_POT_HRSW;
+        s.pot_p[0] = 1.0;
+        s.pot_p[1] = 1.0;
+        s.pot_p[2] = 1.5;
+    } else if (pot == "YUKAWA") {
+        s.potential = JMM_POT_YUKAWA;
+        s.pot_p[0] = 1.0;
+        s.pot_p[1] = 0.2;
+        s.pot_p[2] = 0.5;
+        if (!std::isfinite(s.cutoff)) s.cutoff = 20.0;
+    } else {
+        ok = false;
+    }
+    for (int q = 0; q < inp.potParamsSet; ++q) s.pot_p[q] = inp.potParams[q];
+    if (s.potential == JMM_POT_HRSW) s.cutoff = s.pot_p[2];
+    const bool npt = std::strncmp(inp.ensembleStr, "NPT", 80) == 0;
+    s.ensemble = npt ? JMM_NPT : JMM_NLT;
+    s.boundary = inp.boundary == "PERIODIC" ? JMM_PERIODIC : JMM_WALLS;
+    s.vol_move = inp.volMove == "LOG" ? JMM_VOL_LOG : JMM_VOL_LINEAR;
+    s.nbn = inp.nbn;
+    s.N = inp.N;
+    s.T = inp.T;
+    s.P = inp.P;
+    // NPT starts at l = N (src/jmmMCState.cpp:549-555) unless LSTART is given
+    s.L0 = npt ? (inp.lStart > 0 ? inp.lStart : (double)inp.N) : inp.L;
+    s.max_step = inp.maxStep;
+    s.max_dv = inp.maxdl;
+    s.seed = inp.seed + k;
+    s.chain_id = (uint32_t)k;
+    s.eci = 0;  // the facade's ECheck does the reference's check-and-repair
+    s.bcap = (uint32_t)inp.bcap;
+    return s;
+}
+
+// Runs the queued sweeps and folds their samples (chain 0) into the thermo
+// accumulators.
+static int flush(MCState *m)
+{
+    while (m->pending) {
+        const uint32_t n = (uint32_t)std::min(m->pending, m->cap);
+        int rc = jmm_run_sweeps(m->eng, n);
+        if (rc) return rc;
+        uint32_t got = 0;
+        m->buf.resize((size_t)m->cap * m->specs.size());
+        rc = jmm_read_samples(m->eng, m->buf.data(), (uint32_t)m->cap, &got);
+        if (rc) return rc;
+        double obs[6] = {0, 0, 0, 0, 0, 0};
+        std::vector<double> allobs(6 * m->specs.size());
+        rc = jmm_observables(m->eng, allobs.data(), nullptr);
+        if (rc) return rc;
+        std::copy(allobs.begin(), allobs.begin() + 6, obs);
+        m->Vir = obs[1];
+        m->HV = m->specs[0].potential == JMM_POT_LJ ? m->specs[0].pot_p[1] * obs[5] : 0.0;
+        for (uint32_t s = 0; s < got; ++s) {
+            const jmm_sample &x = m->buf[s];
+            const double rho = (double)m->N / x.L;
+            m->rhoSysA += rho;
+            m->rhoSysSA += rho * rho;
+            m->lA += x.L;
+            m->lSA += x.L * x.L;
+            m->EA += x.U;
+            m->ESA += x.U * x.U;
+            m->lEA += x.L * x.U;
+            m->VirA += m->Vir;
+            m->VirSA += m->Vir * m->Vir;
+            m->EVirA += x.U * m->Vir;
+            m->HVA += m->HV;
+            m->HVSA += m->HV * m->HV;
+            m->nSamples++;
+            m->E = x.U;
+            m->l = x.L;
+        }
+        m->pending -= n;
+        m->sweepsDone += n;
+    }
+    return 0;
+}
+
+static bool crossed(const MCState *m, unsigned long interval)
+{
+    if (!interval || m->sn == m->lastStep) return false;
+    return m->sn / interval != m->lastStep / interval;
+}
+
+struct MCState *setupMCS(struct MCInput inp)
+{
+    if (inp.N < 1) {
+        std::printf("FATAL ERROR: N must be at least 1.\nABORTING SIMULATION\n\n");
+        return nullptr;
+    }
+    MCState *m = new MCState();
+    m->inp = inp;
+    m->N = inp.N;
+    const bool npt = std::strncmp(inp.ensembleStr, "NPT", 80) == 0;
+    m->numTrialTypes = npt ? inp.N + 1 : inp.N;
+    m->inp.numTrialTypes = m->numTrialTypes;
+    m->numSteps = inp.ns;
+    std::printf("ENSEMBLE = %s\n", npt ? "NPT" : "NLT");
+    for (unsigned long k = 0; k < std::max(1ul, inp.chains); ++k) {
+        bool ok;
+        m->specs.push_back(spec_from_input(inp, k, ok));
+        if (!ok) {
+            std::printf("FATAL ERROR: Unknown potential.\nABORTING SIMULATION\n\n");
+            delete m;
+            return nullptr;
+        }
+    }
+    jmm_engine_opts o{};
+    o.form = inp.form == "PLAIN_FP64" ? JMM_FORM_PLAIN_FP64
+           : inp.form == "BATCHED_FP32" ? JMM_FORM_BATCHED_FP32 : JMM_FORM_BATCHED_FP64;
+    o.device = inp.device;
+    o.sample_capacity = (uint32_t)m->cap;
+    o.block_sweeps = (uint32_t)std::max(1ul, inp.sweepBlock);
+    int rc = jmm_create(m->specs.data(), (int)m->specs.size(), &o, &m->eng);
+    if (rc) {
+        die_on(rc, "jmm_create");
+        delete m;
+        return nullptr;
+    }
+    printMCP(m);
+    jmm_counters c{};
+    jmm_get_counters(m->eng, 0, &c);
+    m->E = c.U;
+    m->l = c.L;
+    if (!inp.isRestart) {
+        m->cf = std::fopen("config.dat.mcs", "w");
+        m->tf = std::fopen("thermo.dat.mcs", "w");
+        if (m->tf)
+            std::fprintf(m->tf, "Step    Econf           Econf2          L       L2      LEconf          "
+                                "rho             rho2            Virial         Virial2         EconfVir        "
+                                "HV              HV2 \n");
+    } else {
+        // restart from the last complete frame of config.dat.mcs
+        FILE *f = std::fopen("config.dat.mcs", "r");
+        std::vector<double> x, best;
+        unsigned long step = 0, bestStep = 0;
+        double L = 0, bestL = 0;
+        char line[512];
+        while (f && std::fgets(line, sizeof line, f)) {
+            unsigned long n;
+            if (std::sscanf(line, "%lu", &n) == 1 && !std::strchr(line, ':') && n == m->N &&
+                std::strchr(line, '.') == nullptr) {
+                if (!std::fgets(line, sizeof line, f)) break;
+                if (std::sscanf(line, "Step no.: %lu  Box length: %lf", &step, &L) != 2) continue;
+                x.clear();
+                for (unsigned long i = 0; i < n && std::fgets(line, sizeof line, f); ++i) {
+                    unsigned long pn;
+                    double a, b, z;
+                    if (std::sscanf(line, "%lu %lf %lf %lf", &pn, &a, &b, &z) == 4) x.push_back(z);
+                }
+                if (x.size() == m->N) {
+                    best = x;
+                    bestStep = step;
+                    bestL = L;
+                }
+            }
+        }
+        if (f) std::fclose(f);
+        if (best.empty()) {
+            std::printf("ERROR: no complete configuration in config.dat.mcs\n");
+            freeMCS(m);
+            return nullptr;
+        }
+        std::printf("Complete configuration found for step number %lu\nBox length: %.8g\n", bestStep, bestL);
+        for (size_t k = 0; k < m->specs.size(); ++k) {
+            rc = jmm_set_state(m->eng, (int)k, best.data(), bestL, bestStep / m->numTrialTypes);
+            if (rc) {
+                die_on(rc, "jmm_set_state");
+                freeMCS(m);
+                return nullptr;
+            }
+        }
+        m->sn = m->lastStep = bestStep;
+        jmm_get_counters(m->eng, 0, &c);
+        m->E = c.U;
+        m->l = c.L;
+        m->cf = std::fopen("config.dat.mcs", "a");
+        m->tf = std::fopen("thermo.dat.mcs", "a");
+    }
+    if (inp.mdai && inp.mdai < inp.ns)
+        std::printf("NOTE: DADJ/VADJ step-size adaptation is not run; MAXSTEP and MAXDV stay fixed.\n");
+    return m;
+}
+
+void freeMCS(struct MCState *m)
+{
+    if (!m) return;
+    if (m->cf) std::fclose(m->cf);
+    if (m->tf) std::fclose(m->tf);
+    jmm_destroy(m->eng);
+    delete m;
+}
+
+void printStep(struct MCState *m)
+{
+    std::printf("Step: %lu...\n", m->sn);
+    std::fflush(stdout);
+}
+
+int printMCP(struct MCState *m)
+{
+    const MCInput &i = m->inp;
+    if (i.isRestart) std::printf("*** THIS IS A RESTART RUN ***\n");
+    std::printf("Printing Monte Carlo parameters...\nEnsemble: %s\n", i.ensembleStr);
+    if (m->specs[0].ensemble == JMM_NPT) std::printf("N: %lu\nP: %.5G\nT: %.5G\n", i.N, i.P, i.T);
+    else std::printf("N: %lu\nL: %.5G\nT: %.5G\n", i.N, i.L, i.T);
+    std::printf("Potential: %s\nPotential cut-off: %.5G\n", i.potStr, m->specs[0].cutoff);
+    if (i.nbn > 0) std::printf("Number of neighbors with which each particle can interact: %d\n", i.nbn);
+    else std::printf("No neighbor number limit.\n");
+    std::printf("numSteps: %lu\nmaxStep: %.5G\nmax vol change: %.5G\n", i.ns, i.maxStep, i.maxdl);
+    std::printf("Boundary: %s\nVolume move: %s\nForm: %s\nChains: %zu\n\nSeed: %lu\n\n",
+                i.boundary.c_str(), i.volMove.c_str(), i.form.c_str(), m->specs.size(), i.seed);
+    std::fflush(stdout);
+    return 0;
+}
+
+int fad(struct MCState *m, unsigned long *, double *)
+{
+    // the step-0 full energy (src/Main.cpp:66-68) is computed in FP64 at setup
+    jmm_counters c{};
+    int rc = jmm_get_counters(m->eng, 0, &c);
+    m->E = c.U;
+    return rc;
+}
+
+unsigned long incrementStep(struct MCState *m)
+{
+    if (m->sn >= m->numSteps) return 0;
+    m->lastStep = m->sn;
+    m->sn += m->numTrialTypes;
+    if (m->sn / 10000 != m->lastStep / 10000) printStep(m);
+    return m->sn;
+}
+
+int Step(struct MCState *m)
+{
+    m->pending++;
+    if (m->pending >= m->cap) {
+        if (int rc = flush(m)) return rc;
+    }
+    if (isECheck(m)) ECheck(m);
+    return 0;
+}
+
+int isCoordPrint(struct MCState *m) { return crossed(m, m->inp.cpi); }
+int isThermoPrint(struct MCState *m) { return crossed(m, m->inp.tpi); }
+int isRhoPrint(struct MCState *m) { return crossed(m, m->inp.rhopi); }
+int isGPrint(struct MCState *m) { return crossed(m, m->inp.gpi); }
+int isECheck(struct MCState *m) { return crossed(m, m->inp.eci); }
+int isMaxDisAdjust(struct MCState *m) { return crossed(m, m->inp.mdai); }
+int isMaxDVAdjust(struct MCState *m) { return crossed(m, m->inp.mvai); }
+
+int updateThermo(struct MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    const double rho = (double)m->N / m->l;
+    m->rhoSysA += rho;
+    m->rhoSysSA += rho * rho;
+    m->lA += m->l;
+    m->lSA += m->l * m->l;
+    m->EA += m->E;
+    m->ESA += m->E * m->E;
+    m->lEA += m->l * m->E;
+    m->VirA += m->Vir;
+    m->VirSA += m->Vir * m->Vir;
+    m->EVirA += m->E * m->Vir;
+    m->HVA += m->HV;
+    m->HVSA += m->HV * m->HV;
+    m->nSamples++;
+    return 0;
+}
+
+int printThermo(struct MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    const double ss = m->nSamples ? (double)m->nSamples : 1.0;
+    if (m->tf) {
+        std::fprintf(m->tf, "%lu\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\t%.8G\n",
+                     m->sn, m->EA / ss, m->ESA / ss, m->lA / ss, m->lSA / ss, m->lEA / ss, m->rhoSysA / ss,
+                     m->rhoSysSA / ss, m->VirA / ss, m->VirSA / ss, m->EVirA / ss, m->HVA / ss, m->HVSA / ss);
+        std::fflush(m->tf);
+    }
+    std::printf("%lu  %.8G  %.8G  %.8G  %.8G\n", m->sn, m->E, m->l, m->Vir, m->HV);
+    std::fflush(stdout);
+    m->rhoSysA = m->rhoSysSA = m->lA = m->lSA = m->EA = m->ESA = m->lEA = 0;
+    m->VirA = m->VirSA = m->EVirA = m->HVA = m->HVSA = 0;
+    m->nSamples = 0;
+    return 0;
+}
+
+int printCoords(struct MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    std::vector<double> x(m->N);
+    double L = 0;
+    int rc = jmm_get_state(m->eng, 0, x.data(), &L, nullptr, nullptr);
+    if (rc) return rc;
+    if (!m->cf) return 0;
+    std::fprintf(m->cf, "%lu\nStep no.: %lu  Box length: %.5f\n", m->N, m->sn, L);
+    for (unsigned long i = 0; i < m->N; ++i) std::fprintf(m->cf, "%lu  0.0  0.0  %.8G\n", i, x[i]);
+    std::fflush(m->cf);
+    return 0;
+}
+
+int printRho(struct MCState *) { return 0; }
+int printG(struct MCState *) { return 0; }
+
+int ECheck(struct MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    std::vector<double> Uf(m->specs.size());
+    int rc = jmm_full_energy(m->eng, Uf.data());
+    if (rc) return rc;
+    int result = 0;
+    for (size_t k = 0; k < m->specs.size(); ++k) {
+        jmm_counters c{};
+        jmm_get_counters(m->eng, (int)k, &c);
+        // absolute tolerance of src/jmmMCState.cpp:1998
+        if (std::fabs(Uf[k] - c.U) > 0.0001) {
+            std::printf("\n### !!! $$$ Energy discrepancy!!! ######### !!!!!!!!!!! $$$$$$$$$$ !!!!!!!!\n"
+                        "Chain %zu: E = %.8G, ETest = %.8G, Resetting results to that of full calculation (E = %.8G)\n\n",
+                        k, c.U, Uf[k], Uf[k]);
+            jmm_perturb_energy(m->eng, (int)k, Uf[k] - c.U);
+            result = 1;
+        } else if (k == 0) {
+            std::printf("####### Energy was just verified ########  E = %.8G ######## ETest = %.8G #######\n", c.U, Uf[k]);
+        }
+        if (k == 0) m->E = Uf[0];
+    }
+    return result;
+}
+
+int maxDisAdjust(struct MCState *) { return 0; }
+int maxDVAdjust(struct MCState *) { return 0; }
+
+int relaxVolume(struct MCState *)
+{
+    std::printf("NOTE: RELAX requested; volume relaxation is not part of this engine, skipped.\n");
+    return 0;
+}
+
+unsigned long getStepNum(struct MCState *m) { return m->sn; }
+int getRelaxFlag(struct MCState *m) { return m->inp.relaxFlag; }
+bool getRestartFlag(struct MCState *m) { return m->inp.isRestart; }
+
+int printE(struct MCState *m)
+{
+    flush(m);
+    jmm_counters c{};
+    jmm_get_counters(m->eng, 0, &c);
+    std::printf("\nE = %.8G\n", c.U);
+    return 0;
+}
+
+int printAcc(struct MCState *m)
+{
+    flush(m);
+    jmm_counters c{};
+    jmm_get_counters(m->eng, 0, &c);
+    std::printf("Accepted/Rejected: Displacements VolumeChanges\n              "
+                "        %lu/%lu          %lu/%lu\n",
+                (unsigned long)c.acc_d, (unsigned long)(c.trials - c.acc_d), (unsigned long)c.acc_v,
+                (unsigned long)(c.vol_trials - c.acc_v));
+    return 0;
+}
+
+struct jmm_engine *getEngine(struct MCState *m) { return m->eng; }
+
+double getEnergy(struct MCState *m, int chain)
+{
+    flush(m);
+    jmm_counters c{};
+    jmm_get_counters(m->eng, chain, &c);
+    return c.U;
+}
+
+double getLength(struct MCState *m, int chain)
+{
+    flush(m);
+    jmm_counters c{};
+    jmm_get_counters(m->eng, chain, &c);
+    return c.L;
+}

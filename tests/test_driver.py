@@ -1,0 +1,152 @@
+"""Host side: the INPUT parser (against the reference's own parser test table,
+src/testInput.c:15-63, and the four INPUT files it ships), the C ABI's
+exports, and (GPU) the driver replayed against the oracle.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import oracle_spec, rel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "jmmonedmc_amd", "build", "jmm_onedmc")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "inputs")
+
+
+def parse(path):
+    r = subprocess.run([DRIVER, "--print-input", path], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fields = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line and
+                  not line.startswith("#"))
+    return fields, r.stdout
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(DRIVER):
+        from jmmonedmc_amd import build
+        build.build_driver()
+    return DRIVER
+
+
+def test_reference_parser_table(driver, tmp_path):
+    """The key/value table of src/testInput.c:15-63, written with the same
+    spacing; the unknown POT "YT" falls back to LJ and YADA only warns."""
+    p = tmp_path / "TEST1INPUT"
+    p.write_text("N      5\nP      0.5\nT   0.5\nNUMSTEPS\t\t2\nPOT\t    YT\nNBN\t\t-1\n"
+                 "MAXSTEP    0.1\nMAXDV     1\nCPI\t\t   1\nTPI\t\t1\nRBW\t\t0.01\nRHONB\t\t1000\n"
+                 "RHOPI\t\t1\nGSW\t\t1\nGNS\t\t5\nGBW\t\t0.01\nGNB\t\t1000\nGPI\t\t1\nSEED\t\t125\n"
+                 "YADA\t\t16.543\nENGCHECK\t\t1\nDADJ\t\t1\nVADJ\t\t1")
+    f, out = parse(str(p))
+    want = dict(N="5", ns="2", nbn="-1", cpi="1", tpi="1", rhonb="1000", rhopi="1", gns="5",
+                gnb="1000", gpi="1", seed="125", eci="1", mdai="1", mvai="1", potStr="LJ")
+    for k, v in want.items():
+        assert f[k] == v, (k, f[k])
+    for k, v in dict(P=0.5, T=0.5, maxStep=0.1, maxdl=1.0, rbw=0.01, gsw=1.0, gbw=0.01).items():
+        assert float(f[k]) == v, k
+    assert f["potCutOff"] == "inf"
+    assert "Unknown potential type requested (YT)" in out
+    assert "Property command YADA not understood." in out
+
+
+@pytest.mark.parametrize("name,checks", [
+    ("test_INPUT", dict(ensembleStr="NLT", N="2000", L=4000.0, T=0.5, relaxFlag="1", ns="1000",
+                        potStr="LJcut", potCutOff=5.0, nbn="-1", maxStep=0.1, maxdl=2.0,
+                        rhopi="50000000", seed="774281", eci="1", mdai="10000000")),
+    ("test_INPUTstd", dict(N="10", P=0.7, T=0.4, ns="10", potStr="HARMONIC", nbn="1", seed="125",
+                           gnb="100000", ensembleStr="NPT")),
+    ("test_INPUT_smalltest", dict(N="10", P=1.0, T=0.9, relaxFlag="1", ns="5000000", potStr="LJ",
+                                  seed="92847", eci="1000")),
+    ("scripts_INPUT", dict(N="2000", P=0.7, T=0.8, ns="5000000000", potStr="LJ", nbn="2",
+                           maxdl=5.0, seed="92847", eci="10000")),
+])
+def test_shipped_inputs_parse(driver, name, checks):
+    f, out = parse(os.path.join(GOLDEN, name))
+    for k, v in checks.items():
+        if isinstance(v, float):
+            assert float(f[k]) == v, (k, f[k])
+        else:
+            assert f[k] == v, (k, f[k])
+    if name == "test_INPUTstd":
+        assert "Property command YADA not understood." in out
+
+
+def test_parser_edge_cases(driver, tmp_path):
+    p = tmp_path / "INPUT"
+    p.write_text("# comment line\n\nN 1E3   # scientific notation for an integer\n"
+                 "ENSEMBLE XYZ\nPOT LJshift 2.5\nRESTART false\nBOUNDARY periodic\nVOLMOVE log\n"
+                 "POTPARAMS 1.5 0.9\nCHAINS 4\n" + "T " + "1" * 150 + "\n")
+    f, out = parse(str(p))
+    assert f["N"] == "1000"
+    assert f["ensembleStr"] == "NPT" and "Unknown ensemble type requested (XYZ)" in out
+    assert f["potStr"] == "LJshift" and float(f["potCutOff"]) == 2.5
+    assert f["isRestart"] == "0"
+    assert f["boundary"] == "PERIODIC" and f["volMove"] == "LOG" and f["chains"] == "4"
+    assert f["potParams"].startswith("1.5,0.9") and f["potParamsSet"] == "2"
+    assert float(f["T"]) > 1e149        # a 150-digit value on a long line
+
+
+def test_library_exports_every_declared_symbol():
+    """libjmm_engine.so loads without a GPU and exports every function that
+    include/jmm_engine.h declares."""
+    import ctypes
+    from jmmonedmc_amd import _native
+    hdr = open(os.path.join(ROOT, "include", "jmm_engine.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(jmm_\w+)\s*\(", hdr, re.M))
+    assert declared >= set(_native.EXPORTED)
+    lib = ctypes.CDLL(_native.LIB)
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert lib.jmm_abi_version() == 1
+
+
+def test_schedule_and_validation_without_gpu():
+    """Host-only C ABI calls work without a device: the schedule, and
+    argument validation errors surface through jmm_last_error."""
+    import ctypes as C
+    from jmmonedmc_amd import _native
+    from jmmonedmc_amd.configs import c3
+    from jmmonedmc_amd.engine import Chain, schedule
+    ch = c3(8)
+    assert schedule(ch[0])[0] >= 1
+    lib = _native.lib()
+    bad = Chain(N=10, L0=5.0, cutoff=3.0)           # lattice spacing 0.5 ok, periodic L too short
+    s = bad.to_c()
+    h = C.c_void_p()
+    rc = lib.jmm_create(C.byref(s), 1, None, C.byref(h))
+    assert rc != 0
+    assert b"periodic length" in lib.jmm_last_error() or b"device" in lib.jmm_last_error()
+
+
+@pytest.mark.gpu
+def test_driver_replays_oracle(gpu, oracle, driver, tmp_path):
+    """The driver on a C1-shaped INPUT (periodic, shifted LJ, NVT): the final
+    configuration and energy equal the oracle's replay of the same sweeps."""
+    import pyoracle
+    from jmmonedmc_amd.engine import Chain
+    from helpers import lj
+    inp = tmp_path / "INPUT"
+    inp.write_text("ENSEMBLE NLT\nN 64\nL 128\nT 1.0\nPOT LJshift 3.0\nNBN -1\nMAXSTEP 0.5\n"
+                   "NUMSTEPS 64000\nCPI 64000\nTPI 6400\nSEED 1\nENGCHECK 32000\nBOUNDARY PERIODIC\n"
+                   "FORM BATCHED_FP64\n")
+    print("[gpu] driver C1 1000 sweeps", flush=True)
+    r = subprocess.run([driver, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "PROGRAM COMPLETED SUCCESSFULLY!" in r.stdout
+    assert "Energy was just verified" in r.stdout
+    E = float(re.search(r"\nE = (\S+)", r.stdout).group(1))
+    ch = Chain(N=64, L0=128.0, T=1.0, seed=1, chain_id=0, **lj())
+    c = oracle.Chain(oracle_spec(pyoracle, ch))
+    c.sweeps(1000)
+    x, L, U, _ = c.state()
+    assert rel(E, U) <= 1e-7              # printed with %.8G
+    frames = (tmp_path / "config.dat.mcs").read_text().split("Step no.:")
+    last = [float(l.split()[3]) for l in frames[-1].splitlines()[1:] if l.strip() and len(l.split()) == 4]
+    assert len(last) == 64
+    assert np.allclose(last, x, rtol=1e-7, atol=1e-6)
+    rows = (tmp_path / "thermo.dat.mcs").read_text().splitlines()
+    assert rows[0].startswith("Step") and len(rows) == 1 + 1 + 10   # header, step 0, 10 blocks
+    assert len(rows[-1].split("\t")) == 13

@@ -54,7 +54,7 @@ def case_lj_nbn1_walls(seed=3, chain_id=0):
 def case_lj_nbn1_periodic_log(seed=4, chain_id=0):
     # shifted LJ rc=3 with NBN 1 on a ring, ln L moves
     shift = 4 * (3.0 ** -12 - 3.0 ** -6)
-    ch = Chain(N=24, rho=0.6, T=0.8, P=0.3, ensemble=nat.NPT, vol_move=nat.VOL_LOG, max_dv=0.03,
+    ch = Chain(N=24, rho=0.31, T=0.8, P=0.3, ensemble=nat.NPT, vol_move=nat.VOL_LOG, max_dv=0.03,
                max_step=0.4, nbn=1, seed=seed, chain_id=chain_id, **lj())
     mom = gap_moments(lj_phi(rc=3.0, shift=shift), 1 / 0.8, 0.3, 0.5, breaks=(1.0, 1.5, 3.0))
     return ch, periodic_npt(24, mom)
@@ -96,7 +96,7 @@ def test_engine_matches_exact_npt(gpu, name, form):
         chains.append(ch)
     print(f"[gpu] exact {name} {form}", flush=True)
     eng = Engine(chains, form=form, sample_capacity=4000, block_sweeps=500)
-    eng.run(1000)
+    eng.run(3000)                       # equilibrate from the lattice start
     eng.read_samples(4000)
     eng.run(4000)
     smp = eng.read_samples(4000)
