@@ -72,6 +72,7 @@ void schedule_of(const jmm_chain_spec &s, uint64_t &B, uint64_t &M, uint64_t &w)
         if ((uint64_t)s.nbn < wc) wc = (uint64_t)s.nbn;
         if ((uint64_t)s.nbn + 1 < mc) mc = (uint64_t)s.nbn + 1;
     }
+    if (mc < 2) mc = 2;  // the hard-core test reads both nearest partners
     if (N >= 1 && wc > N - 1) wc = N - 1;
     uint64_t nb = 1;
     if (mc >= 1 && mc <= N) {
@@ -202,6 +203,9 @@ DevChain make_chain(const jmm_chain_spec &s)
     c.B = (uint32_t)B;
     c.w = (uint32_t)w;
     c.smax = (uint32_t)((s.N + B - 1) / B);
+    c.wg = c.w > 0 ? c.w : 1;
+    c.Sp = c.smax | 1u;
+    c.lds_len = 2 * c.wg + c.B * c.Sp;
     c.chain_id = s.chain_id;
     c.eci = s.eci;
     c.flags = (s.boundary == JMM_PERIODIC ? F_PERIODIC : 0u) | (s.ensemble == JMM_NPT ? F_NPT : 0u) |
@@ -233,6 +237,7 @@ int dalloc(T **p, size_t n)
 
 struct Group {
     int gx = 0;
+    int kind = 0;
     uint32_t n_items = 0;     // bundles (batched) or chains (plain)
     uint32_t grid = 0;
     size_t lds = 0;
@@ -259,7 +264,7 @@ struct jmm_engine {
     double *d_obs = nullptr, *d_grad = nullptr, *d_Ltmp = nullptr;
     uint32_t *d_all = nullptr;
     uint32_t cap = 1024, write_pos = 0, block_sweeps = 100;
-    Group grp[2];
+    std::vector<Group> grp;
     int use_graph = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -271,7 +276,7 @@ namespace {
 
 int launch_block(jmm_engine *e, uint32_t m)
 {
-    for (int g = 0; g < 2; ++g) {
+    for (size_t g = 0; g < e->grp.size(); ++g) {
         Group &G = e->grp[g];
         if (!G.n_items) continue;
         HIPCHK(launch_begin(&e->d_ctrl[g], m, e->stream));
@@ -321,14 +326,18 @@ struct PackOut {
 };
 
 PackOut pack(const std::vector<jmm_chain_spec> &specs, const std::vector<DevChain> &ch, int T,
-             size_t lds_doubles)
+             size_t lds_doubles, size_t lds_max_doubles)
 {
     const int n = (int)specs.size();
     std::vector<uint32_t> idx(n);
     std::iota(idx.begin(), idx.end(), 0u);
+    // chains of one bundle share everything but N, T, P, L0, seed and stream
+    // (the kernel reads the shared part once per bundle)
     auto key = [&](uint32_t c) {
         const jmm_chain_spec &s = specs[c];
-        return std::make_tuple(s.potential, s.ensemble, s.boundary, s.eci);
+        return std::make_tuple(s.potential, s.ensemble, s.boundary, s.eci, s.vol_move, s.nbn,
+                               s.shifted, s.max_step, s.max_dv, s.cutoff, s.pot_p[0], s.pot_p[1],
+                               s.pot_p[2], s.pot_p[3]);
     };
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
         if (key(a) != key(b)) return key(a) < key(b);
@@ -339,7 +348,7 @@ PackOut pack(const std::vector<jmm_chain_spec> &specs, const std::vector<DevChai
     PackOut o;
     // global-x chains first (they run in their own group), then LDS bundles
     std::vector<uint32_t> gx, lds;
-    for (uint32_t c : idx) ((size_t)ch[c].N > lds_doubles ? gx : lds).push_back(c);
+    for (uint32_t c : idx) ((size_t)ch[c].lds_len > lds_max_doubles ? gx : lds).push_back(c);
     uint64_t xoff = 0;
     auto open_bundle = [&](int g, uint32_t c) {
         DevBundle bd{};
@@ -367,7 +376,7 @@ PackOut pack(const std::vector<jmm_chain_spec> &specs, const std::vector<DevChai
             const DevBundle &bd = o.b[0].back();
             const uint32_t c0 = o.order[o.slots[bd.slot0].chain];
             fresh = key(c0) != key(c) || bd.lanes + ch[c].B > (uint32_t)T ||
-                    (size_t)bd.lds_doubles + ch[c].N > lds_doubles;
+                    (size_t)bd.lds_doubles + ch[c].lds_len > lds_doubles;
         }
         if (fresh) open_bundle(0, c);
         DevBundle &bd = o.b[0].back();
@@ -375,7 +384,7 @@ PackOut pack(const std::vector<jmm_chain_spec> &specs, const std::vector<DevChai
         bd.nslots++;
         bd.lanes += ch[c].B;
         bd.smax = std::max(bd.smax, ch[c].smax);
-        bd.lds_doubles += ch[c].N;
+        bd.lds_doubles += ch[c].lds_len;
         o.order.push_back(c);
         xoff += ch[c].N;
     }
@@ -489,7 +498,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         if (maxB > 1024) return bail(fail(JMM_EINVAL, "jmm_create: more than 1024 blocks in a chain"));
         size_t budget = o.lds_budget ? o.lds_budget : 65536;
         budget = std::min<size_t>(budget, (size_t)lds_max);
-        PackOut p = pack(e->specs, ch, e->T, budget / 8);
+        PackOut p = pack(e->specs, ch, e->T, budget / 8, (size_t)lds_max / 8 - 64);
         order = p.order;
         slots = p.slots;
         for (int g = 0; g < 2; ++g) bundles[g] = p.b[g];
@@ -528,7 +537,6 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     if (int rc = dalloc(&e->d_chains, n_chains)) return bail(rc);
     if (int rc = dalloc(&e->d_states, n_chains)) return bail(rc);
     if (int rc = dalloc(&e->d_x, e->total_N)) return bail(rc);
-    if (int rc = dalloc(&e->d_ctrl, 2)) return bail(rc);
     if (int rc = dalloc(&e->d_sU, (size_t)e->cap * n_chains)) return bail(rc);
     if (int rc = dalloc(&e->d_sL, (size_t)e->cap * n_chains)) return bail(rc);
     if (int rc = dalloc(&e->d_sA, (size_t)e->cap * n_chains)) return bail(rc);
@@ -537,10 +545,8 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     HIPCHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_states, hst.data(), sizeof(DevState) * n_chains, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
-    Ctrl zc[2] = {};
-    HIPCHK(hipMemcpy(e->d_ctrl, zc, sizeof zc, hipMemcpyHostToDevice));
-
-    // launch groups
+    // launch groups: plain form by LDS fit; batched forms by (memory mode,
+    // potential), one persistent launch each
     if (e->form == JMM_FORM_PLAIN_FP64) {
         std::vector<uint32_t> lists[2];
         size_t maxlds[2] = {0, 0};
@@ -551,30 +557,31 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             if (!g) maxlds[0] = std::max(maxlds[0], need);
         }
         for (int g = 0; g < 2; ++g) {
-            Group &G = e->grp[g];
+            if (lists[g].empty()) continue;
+            Group G;
             G.gx = g;
             G.n_items = (uint32_t)lists[g].size();
             G.grid = G.n_items;
             G.lds = maxlds[g];
-            if (!G.n_items) continue;
-            if (int rc = set_lds_attr(plain_kernel(g), G.lds)) return bail(rc);
-            if (int rc = dalloc(&G.d_list, G.n_items)) return bail(rc);
-            HIPCHK(hipMemcpy(G.d_list, lists[g].data(), sizeof(uint32_t) * G.n_items, hipMemcpyHostToDevice));
+            e->grp.push_back(G);
+            Group &Gr = e->grp.back();
+            if (int rc = set_lds_attr(plain_kernel(g), Gr.lds)) return bail(rc);
+            if (int rc = dalloc(&Gr.d_list, Gr.n_items)) return bail(rc);
+            HIPCHK(hipMemcpy(Gr.d_list, lists[g].data(), sizeof(uint32_t) * Gr.n_items, hipMemcpyHostToDevice));
         }
     } else {
-        // slots refer to internal chain indices already (pack assigned them in order)
         if (int rc = dalloc(&e->d_slots, slots.size())) return bail(rc);
         HIPCHK(hipMemcpy(e->d_slots, slots.data(), sizeof(DevSlot) * slots.size(), hipMemcpyHostToDevice));
         for (int g = 0; g < 2; ++g) {
-            Group &G = e->grp[g];
-            G.gx = g;
-            std::vector<DevBundle> &bv = bundles[g];
-            // longest-processing-time first
+            std::vector<DevBundle> bv = bundles[g];
+            if (bv.empty()) continue;
+            // longest-processing-time first, all potentials in one queue
             std::stable_sort(bv.begin(), bv.end(), [&](const DevBundle &a, const DevBundle &b) {
                 return bundle_cost(a, e->hch, slots) > bundle_cost(b, e->hch, slots);
             });
+            Group G;
+            G.gx = g;
             G.n_items = (uint32_t)bv.size();
-            if (!G.n_items) continue;
             size_t maxd = 0;
             for (auto &b : bv) maxd = std::max<size_t>(maxd, b.lds_doubles);
             G.lds = g ? 0 : maxd * sizeof(double);
@@ -586,7 +593,13 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             G.grid = std::min<uint32_t>(G.n_items, (uint32_t)(occ * n_cu));
             if (int rc = dalloc(&G.d_bundles, G.n_items)) return bail(rc);
             HIPCHK(hipMemcpy(G.d_bundles, bv.data(), sizeof(DevBundle) * G.n_items, hipMemcpyHostToDevice));
+            e->grp.push_back(G);
         }
+    }
+    if (int rc = dalloc(&e->d_ctrl, std::max<size_t>(1, e->grp.size()))) return bail(rc);
+    {
+        std::vector<Ctrl> zc(std::max<size_t>(1, e->grp.size()));
+        HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
     }
 
     if (e->use_graph) {
@@ -686,8 +699,8 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
             o.acc_v = A[r].y;
         }
     e->write_pos = 0;
-    Ctrl zc[2] = {};
-    HIPCHK(hipMemcpy(e->d_ctrl, zc, sizeof zc, hipMemcpyHostToDevice));
+    std::vector<Ctrl> zc(std::max<size_t>(1, e->grp.size()));
+    HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
     *n_out = n;
     return 0;
 }

@@ -153,25 +153,37 @@ JMM_HD float fexp(float x)
 #endif
 }
 
-// FP32 pair energy of an FP64-formed separation d whose cutoff test `in` was
-// made in FP64.  Hard cores are handled before any pair is evaluated.
+// FP32 pair energy of a separation d whose cutoff test `in` the caller has
+// made; evaluated without branches and selected, so that a wavefront keeps
+// one instruction stream.  Hard cores are handled before any pair is
+// evaluated.
 template <int KIND>
 JMM_HD float phi32(const Pot &q, float d, bool in)
 {
-    if (!in) return 0.0f;
+    float v;
     if (KIND == POT_LJ) {
         const float ir2 = frcp(d * d);
         const float s6 = ir2 * ir2 * ir2 * q.sig6f;
-        return q.e4f * s6 * (s6 - 1.0f) - q.shiftf;
+        v = q.e4f * s6 * (s6 - 1.0f) - q.shiftf;
     } else if (KIND == POT_HARMONIC) {
-        if (d <= 0.0f) return 10e10f;
         const float m = d - 1.0f;
-        return m * m;
+        v = d <= 0.0f ? 10e10f : m * m;
     } else if (KIND == POT_HRSW) {
-        return -q.p1f;
+        v = -q.p1f;
     } else {
-        return q.p0f * fexp(-q.p1f * d) * frcp(d) - q.shiftf;
+        v = q.p0f * fexp(-q.p1f * d) * frcp(d) - q.shiftf;
     }
+    return in ? v : 0.0f;
+}
+
+// FP32 inside-the-cutoff test (strict for harmonic and the square well,
+// inclusive otherwise).
+template <int KIND>
+JMM_HD bool inside32(const Pot &q, float d)
+{
+    if (KIND == POT_HRSW) return d < q.p2f;
+    if (KIND == POT_HARMONIC) return d < q.rcf;
+    return d <= q.rcf;
 }
 
 // Pair observables for the backward kernel: E, W = -d phi', dE/dtheta0,

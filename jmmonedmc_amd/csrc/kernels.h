@@ -14,8 +14,12 @@
 // t every block updates its t-th particle.  Trials in one phase are at least
 // M_min apart in index order, and M_min is chosen so that no two of them can
 // interact or read each other's particle (hard core g_min, cutoff rc, step
-// delta: M_min = floor((rc + 2 delta)/g_min) + 1; or NBN + 1).  A phase is
+// delta: M_min = floor((rc + 2 delta)/g_min) + 1, or NBN + 1, and at least 2
+// because the hard-core test reads both nearest partners).  A phase is
 // therefore equivalent to running its trials one after another in any order.
+//
+// LDS layout of a chain inside a bundle (batched forms): ghost-left (wg),
+// B blocks at an odd stride Sp, ghost-right (wg); see LdsChain.
 #pragma once
 
 #include "jmm_device.h"
@@ -30,6 +34,9 @@ struct DevChain {
     uint64_t log_off; // first byte of this chain's accept log
     uint32_t N, B, w, smax;
     uint32_t chain_id, eci, flags, kind;
+    uint32_t wg, Sp;  // LDS layout: ghost width max(w, 1), odd block stride >= smax
+    uint32_t lds_len; // LDS doubles of the chain's region: 2 wg + B Sp
+    uint32_t pad0;
     double beta, P, max_step, max_dv, gmin, Lmin;
     double sh[3];  // shifted form at the cutoff: E, dE/dtheta0, dE/dtheta1
     Pot pot;

@@ -16,6 +16,11 @@
 //   k_begin          resets a work queue and advances the sample row.
 #include "kernels.h"
 
+// Occupancy target of the batched kernels (waves per SIMD); bounds VGPRs.
+#ifndef JMM_WAVES_PER_EU
+#define JMM_WAVES_PER_EU 2
+#endif
+
 namespace jmm {
 
 __device__ __forceinline__ bool nbpos(const double *X, int64_t j, int64_t N, bool per, double L,
@@ -31,8 +36,9 @@ __device__ __forceinline__ bool nbpos(const double *X, int64_t j, int64_t N, boo
 }
 
 // Energy change for moving the particle at xi (index i) to xn.  Partners:
-// left side nearest first, then right side, stopping at the first partner
-// beyond the cutoff from both positions or at index distance w.  Restates the
+// left side nearest first and right side nearest first, each stopping at the
+// first partner beyond the cutoff from both positions or at index distance
+// w; the two side sums are added at the end.  Restates the
 // partner sums of qad2 (src/jmmMCState.cpp:1213-1348), whose far pairs
 // contribute exactly zero.
 template <int KIND>
@@ -40,13 +46,13 @@ __device__ __forceinline__ double delta_e64(const Pot &q, const double *X, int64
                                             bool per, double L, uint32_t w, double xi, double xn,
                                             uint32_t &terms)
 {
-    double dE = 0.0;
+    double dEL = 0.0, dER = 0.0;
     for (uint32_t k = 1; k <= w; ++k) {
         double xj;
         if (!nbpos(X, i - (int64_t)k, N, per, L, xj)) break;
         const double dold = xi - xj, dnew = xn - xj;
         if (dold > q.rc && dnew > q.rc) break;
-        dE += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
+        dEL += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
         terms += 2;
     }
     for (uint32_t k = 1; k <= w; ++k) {
@@ -54,10 +60,10 @@ __device__ __forceinline__ double delta_e64(const Pot &q, const double *X, int64
         if (!nbpos(X, i + (int64_t)k, N, per, L, xj)) break;
         const double dold = xj - xi, dnew = xj - xn;
         if (dold > q.rc && dnew > q.rc) break;
-        dE += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
+        dER += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
         terms += 2;
     }
-    return dE;
+    return dEL + dER;
 }
 
 // FP32 form: separations formed in FP64 (positions are FP64), cutoff decided
@@ -81,6 +87,9 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
         s = tt;
         terms += 2;
     }
+    const double left = (double)s - (double)c;
+    s = 0.0f;
+    c = 0.0f;
     for (uint32_t k = 1; k <= w; ++k) {
         double xj;
         if (!nbpos(X, i + (int64_t)k, N, per, L, xj)) break;
@@ -94,7 +103,7 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
         s = tt;
         terms += 2;
     }
-    return (double)s - (double)c;
+    return left + ((double)s - (double)c);
 }
 
 // Pair-by-pair energy change of scaling every coordinate by sc (box-length
@@ -182,7 +191,9 @@ struct VolProp {
     bool valid;
 };
 
-__device__ __forceinline__ VolProp vol_propose(const DevChain &ch, double L, uint64_t sweep)
+// Out of line: its FP64 log/exp/division temporaries would otherwise raise
+// the register count of the whole sweep loop.
+__device__ __noinline__ VolProp vol_propose(const DevChain &ch, double L, uint64_t sweep)
 {
     uint32_t r[4];
     philox4x32_10(0xFFFFFFFFu, (uint32_t)sweep, ch.chain_id, 0u, (uint32_t)ch.seed,
@@ -276,9 +287,9 @@ struct BatchedScratch {
     uint32_t item;
 };
 
-template <typename Real, int KIND, int T, bool GX>
-__device__ void run_bundle(const SweepArgs &a, const DevBundle &bd, double *smem,
-                           BatchedScratch<T> &sc, uint32_t soff)
+template <typename Real, int KIND, int T>
+__device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundle &bd, BatchedScratch<T> &sc,
+                              uint32_t soff)
 {
     const uint32_t tid = threadIdx.x;
     // lane -> slot: the largest slot whose first lane is <= tid
@@ -292,14 +303,7 @@ __device__ void run_bundle(const SweepArgs &a, const DevBundle &bd, double *smem
     const bool act = tid >= me.lane_off && tid < me.lane_off + me.B;
     const uint32_t ic = me.chain;
 
-    double *X;
-    if (GX) {
-        X = a.x + bd.x_off;
-    } else {
-        const double *xg = a.x + bd.x_off;
-        for (uint32_t k = tid; k < bd.lds_doubles; k += T) smem[k] = xg[k];
-        X = smem + me.lds_off;
-    }
+    double *X = a.x + bd.x_off;
     __syncthreads();
 
     const DevChain &ch = a.chains[ic];
@@ -451,16 +455,368 @@ __device__ void run_bundle(const SweepArgs &a, const DevBundle &bd, double *smem
         }
     }
     __syncthreads();
-    if (!GX) {
-        double *xg = a.x + bd.x_off;
-        for (uint32_t k = tid; k < bd.lds_doubles; k += T) xg[k] = smem[k];
+}
+
+// Per-lane view of a chain held in LDS.  Region layout:
+//   [ghost-left: wg][block 0 .. block B-1 at stride Sp][ghost-right: wg]
+// Sp is odd and at least the longest block, so the lanes of one chain, which
+// touch the same offset of consecutive blocks in a phase, hit distinct LDS
+// banks.  Periodic chains keep images of the particles near the ends:
+// ghost-left[m] = x_{N-wg+m} - L, ghost-right[m] = x_m + L (the same FP64
+// operation the oracle applies on the fly), so a neighbour walk never
+// branches on the wrap.  wg = max(w, 1).
+struct LdsChain {
+    double *R;
+    uint32_t N, B, w, wg, Sp, qb, rem;
+    bool per;
+};
+
+__device__ __forceinline__ uint32_t lds_addr(const LdsChain &c, uint32_t i)
+{
+    const uint32_t big = c.rem * (c.qb + 1);
+    uint32_t b, t;
+    if (i < big) {
+        b = i / (c.qb + 1);
+        t = i - b * (c.qb + 1);
+    } else {
+        b = c.rem + (i - big) / c.qb;
+        t = i - (b * c.qb + c.rem);
+    }
+    return c.wg + b * c.Sp + t;
+}
+
+// Images of the first and last wg particles; lanes r = 0..B-1 share the work.
+__device__ __forceinline__ void fill_ghosts(const LdsChain &c, uint32_t r, double L)
+{
+    if (!c.per) return;
+    for (uint32_t m = r; m < c.wg && m < c.N; m += c.B) {
+        c.R[c.wg + c.B * c.Sp + m] = c.R[lds_addr(c, m)] + L;
+        c.R[m] = c.R[lds_addr(c, c.N - c.wg + m)] - L;
     }
 }
 
-// Persistent kernel: every workgroup pulls bundles from the queue until it is
-// empty; all waves of a workgroup leave together when the queue is drained.
+// One displacement trial of particle i (block position t of a block of
+// length size, LDS address my) by md.  Same decisions as orc_trial_with: wall
+// test, hard core against the nearest partners (FP64), then the partner sums
+// of qad2 (src/jmmMCState.cpp:1213-1348) walked outward on both sides at once
+// in a wave-uniform loop; each side stops at its first partner beyond the
+// cutoff from both positions.  FP64 form: the oracle's operations exactly.
+// FP32 form: separations formed in FP64, cutoff decided in FP64 (the FP64
+// set-up and recompute energies count the same pairs), pair energies in FP32
+// without branches, Kahan-compensated side sums; the loop-exit test runs on
+// the FP32 roundings (it only ends the walk, a pair it keeps adds zero).  Returns the decision; dE receives
+// the energy change.
+template <typename Real, int KIND>
+__device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, double gmin,
+                                          double beta, float betaf, uint32_t i, uint32_t t,
+                                          uint32_t size, uint32_t my, uint32_t gapL,
+                                          uint32_t gapR, double L, double xi, double xn, double md,
+                                          double u, double &dE, uint32_t &terms, uint32_t &n_wall,
+                                          uint32_t &n_core)
+{
+    const double *X = c.R;
+    if (!c.per && fabs(xn) > L / 2.0) {
+        ++n_wall;
+        return false;
+    }
+    const bool hasL = c.N >= 2 && (c.per || i > 0);
+    const bool hasR = c.N >= 2 && (c.per || i + 1 < c.N);
+    const double xl = X[hasL ? my - 1 - (t == 0 ? gapL : 0) : my];
+    const double xr = X[hasR ? my + 1 + (t + 1 >= size ? gapR : 0) : my];
+    if ((hasL && (xn - xl) < gmin) || (hasR && (xr - xn) < gmin)) {
+        ++n_core;
+        return false;
+    }
+    const uint32_t kL = c.per ? c.w : min(c.w, i);
+    const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+    bool liveL = kL >= 1, liveR = kR >= 1;
+    if (sizeof(Real) == 8) {
+        double dEL = 0.0, dER = 0.0;
+#pragma unroll 1
+        for (uint32_t k = 1; __any(liveL || liveR); ++k) {
+            const double xjL = k == 1 ? xl : X[liveL ? my - k - (k > t ? gapL : 0) : my];
+            const double xjR = k == 1 ? xr : X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
+            const double oL = xi - xjL, nL = xn - xjL;
+            const double oR = xjR - xi, nR = xjR - xn;
+            const bool inL = liveL && !(oL > q.rc && nL > q.rc);
+            const bool inR = liveR && !(oR > q.rc && nR > q.rc);
+            const double tL = phi64<KIND>(q, nL) - phi64<KIND>(q, oL);
+            const double tR = phi64<KIND>(q, nR) - phi64<KIND>(q, oR);
+            if (inL) dEL += tL;
+            if (inR) dER += tR;
+            terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
+            liveL = inL && k < kL;
+            liveR = inR && k < kR;
+        }
+        dE = dEL + dER;
+        if (dE <= 0.0) return true;
+        return exp(-dE * beta) > u;
+    }
+    float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
+#pragma unroll 1
+    for (uint32_t k = 1; __any(liveL || liveR); ++k) {
+        const double xjL = k == 1 ? xl : X[liveL ? my - k - (k > t ? gapL : 0) : my];
+        const double xjR = k == 1 ? xr : X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
+        // separations are FP64 differences; the cutoff decision is made on
+        // them (the FP64 energies of set-up and recompute count the same
+        // pairs), the pair energies on their FP32 roundings
+        const double oL = xi - xjL, nL = xn - xjL;
+        const double oR = xjR - xi, nR = xjR - xn;
+        const float oLf = (float)oL, nLf = (float)nL, oRf = (float)oR, nRf = (float)nR;
+        const bool inL = liveL && !(oLf > q.rcf && nLf > q.rcf);
+        const bool inR = liveR && !(oRf > q.rcf && nRf > q.rcf);
+        const float tL = phi32<KIND>(q, nLf, inside64<KIND>(q, nL)) -
+                         phi32<KIND>(q, oLf, inside64<KIND>(q, oL));
+        const float tR = phi32<KIND>(q, nRf, inside64<KIND>(q, nR)) -
+                         phi32<KIND>(q, oRf, inside64<KIND>(q, oR));
+        const float yL = (inL ? tL : 0.0f) - cL;
+        const float uL = sL + yL;
+        cL = (uL - sL) - yL;
+        sL = uL;
+        const float yR = (inR ? tR : 0.0f) - cR;
+        const float uR = sR + yR;
+        cR = (uR - sR) - yR;
+        sR = uR;
+        terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
+        liveL = inL && k < kL;
+        liveR = inR && k < kR;
+    }
+    dE = ((double)sL - (double)cL) + ((double)sR - (double)cR);
+    if (dE <= 0.0) return true;
+    return (double)fexp(-betaf * (float)dE) > u;
+}
+
+template <typename Real, int KIND, int T>
+__device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBundle &bd, double *smem,
+                                            BatchedScratch<T> &sc, uint32_t soff)
+{
+    const uint32_t tid = threadIdx.x;
+    uint32_t lo = 0, hi = bd.nslots - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a.slots[bd.slot0 + mid].lane_off <= tid) lo = mid;
+        else hi = mid - 1;
+    }
+    const DevSlot me = a.slots[bd.slot0 + lo];
+    const bool act = tid >= me.lane_off && tid < me.lane_off + me.B;
+    const uint32_t ic = me.chain;
+    const DevChain &ch = a.chains[ic];
+    // Chains of a bundle share their shape (potential and its parameters,
+    // boundary, ensemble, step sizes, window): read it once per bundle.
+    const DevChain &sh = a.chains[__builtin_amdgcn_readfirstlane(a.slots[bd.slot0].chain)];
+    const Pot q = sh.pot;
+    const double gmin = sh.gmin, max_step = sh.max_step;
+    const double beta = ch.beta;
+    const float betaf = ch.betaf;
+    const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+
+    LdsChain c;
+    c.R = smem + me.lds_off;
+    c.N = ch.N;
+    c.B = me.B;
+    c.w = sh.w;
+    c.wg = sh.wg;
+    c.Sp = ch.Sp;
+    c.qb = ch.N / me.B;
+    c.rem = ch.N % me.B;
+    c.per = (sh.flags & F_PERIODIC) != 0;
+    const bool npt = (sh.flags & F_NPT) != 0;
+
+    const uint32_t r = tid - me.lane_off;
+    const uint32_t size = act ? c.qb + (r < c.rem ? 1u : 0u) : 0u;
+    const uint32_t start = act ? r * c.qb + (r < c.rem ? r : c.rem) : 0u;
+    const uint32_t a0 = c.wg + r * c.Sp;
+    const uint32_t gapR = c.Sp - size;
+    const uint32_t gapL = r == 0 ? 0u : c.Sp - (c.qb + (r - 1 < c.rem ? 1u : 0u));
+    const uint32_t seg = act ? c.B : 1u;
+    const bool leader = act && r == 0;
+
+    const double *xg = a.x + ch.x_off;
+    for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = xg[start + t];
+    const DevState st = a.states[ic];
+    double L = st.L, U = st.U, mdrift = st.max_drift;
+    uint64_t sweep = st.sweep;
+    __syncthreads();
+    if (act) fill_ghosts(c, r, L);
+    __syncthreads();
+
+    uint32_t n_tr = 0, n_wall = 0, n_core = 0, n_terms = 0;
+    uint32_t acc_total = 0, n_vt = 0, n_av = 0, n_rc = 0;
+
+#pragma unroll 1
+    for (uint32_t sw = 0; sw < a.n_sweeps; ++sw, ++sweep) {
+        double dUs = 0.0;
+        uint32_t acc = 0;
+#pragma unroll 1
+        for (uint32_t t = 0; t < bd.smax; ++t) {
+            if (t < size) {
+                const uint32_t i = start + t;
+                const uint32_t my = a0 + t;
+                uint32_t rr[4];
+                philox4x32_10(i, (uint32_t)sweep, ch.chain_id, 0u, key0, key1, rr);
+                const double a1 = u53(rr[0], rr[1]);
+                const double u = u53(rr[2], rr[3]);
+                const double md = ((a1 - 0.5) * 2.0) * max_step;
+                const double xi = c.R[my];
+                const double xn = xi + md;
+                ++n_tr;
+                double dE = 0.0;
+                const bool accd = trial_lds<Real, KIND>(c, q, gmin, beta, betaf, i, t, size, my,
+                                                        gapL, gapR, L, xi, xn, md, u, dE, n_terms,
+                                                        n_wall, n_core);
+                if (accd) {
+                    c.R[my] = xn;
+                    if (c.per) {
+                        if (i < c.wg) c.R[c.wg + c.B * c.Sp + i] = xn + L;
+                        if (i + c.wg >= c.N) c.R[i + c.wg - c.N] = xn - L;
+                    }
+                    dUs += dE;
+                    ++acc;
+                }
+                if (a.acclog) a.acclog[ch.log_off + (uint64_t)(soff + sw) * c.N + i] = accd ? 1 : 0;
+            }
+            __syncthreads();
+        }
+
+        // box-length trial (fav, src/jmmMCState.cpp:2161-2293): energy change
+        // pair by pair over the pairs whose left member is in this block
+        VolProp vp{};
+        uint32_t bad = 0;
+        double vpart = 0.0;
+        if (npt) {
+            vp = vol_propose(ch, L, sweep);
+            if (vp.valid && act) {
+                double part = 0.0;
+                float s = 0.0f, cc = 0.0f;
+#pragma unroll 1
+                for (uint32_t t = 0; t < size && !bad; ++t) {
+                    const uint32_t i = start + t;
+                    const uint32_t my = a0 + t;
+                    const double xi = c.R[my], xis = xi * vp.sc;
+                    const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+#pragma unroll 1
+                    for (uint32_t k = 1; k <= kR; ++k) {
+                        const bool cross = t + k >= size;
+                        const double xj = c.R[my + k + (cross ? gapR : 0)];
+                        double xjs;
+                        if (cross && r + 1 == c.B) xjs = c.R[c.wg + (t + k - size)] * vp.sc + vp.Lp;
+                        else xjs = xj * vp.sc;
+                        const double d0 = xj - xi, d1 = xjs - xis;
+                        if (k == 1 && d1 < gmin) {
+                            bad = 1;
+                            break;
+                        }
+                        if (d0 > q.rc && d1 > q.rc) break;
+                        if (sizeof(Real) == 8) {
+                            part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
+                        } else {
+                            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
+                                             phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
+                            const float y = tt - cc;
+                            const float z = s + y;
+                            cc = (z - s) - y;
+                            s = z;
+                        }
+                    }
+                }
+                vpart = sizeof(Real) == 8 ? part : (double)s - (double)cc;
+            }
+        }
+        {
+            double v[2] = {dUs, vpart};
+            uint32_t k2[2] = {acc, bad};
+            seg_allreduce<T, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            U = U + v[0];
+            vpart = v[1];
+            acc = k2[0];
+            bad = k2[1];
+        }
+        acc_total += acc;
+        uint32_t accv = 0;
+        if (npt) {
+            ++n_vt;
+            if (vp.valid && !bad && vol_accept(vpart, vp.dL, vp.lns, ch.P, beta, vp.jac, vp.u)) {
+                for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
+                L = vp.Lp;
+                U = U + vpart;
+                accv = 1;
+                ++n_av;
+            }
+            __syncthreads();
+            // only the chain's own lanes: idle lanes of the workgroup carry no
+            // energy partials and so no valid decision
+            if (accv && act) fill_ghosts(c, r, L);
+            __syncthreads();
+        }
+
+        // drift guard (ECheck, src/jmmMCState.cpp:1965-2095), always FP64
+        const int need = act && ch.eci && ((sweep + 1) % ch.eci == 0);
+        if (__syncthreads_or(need)) {
+            double Up = 0.0;
+            if (need) {
+                for (uint32_t t = 0; t < size; ++t) {
+                    const uint32_t i = start + t;
+                    const uint32_t my = a0 + t;
+                    const double xi = c.R[my];
+                    const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+                    for (uint32_t k = 1; k <= kR; ++k) {
+                        const double d = c.R[my + k + (t + k >= size ? gapR : 0)] - xi;
+                        if (d > q.rc) break;
+                        Up += phi64<KIND>(q, d);
+                    }
+                }
+            }
+            double v[1] = {Up};
+            uint32_t k1[1] = {0};
+            seg_allreduce<T, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            if (need) {
+                const double dr = fabs(v[0] - U);
+                if (dr > mdrift) mdrift = dr;
+                U = v[0];
+                ++n_rc;
+            }
+        }
+
+        if (leader) {
+            const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
+            a.sU[row] = U;
+            a.sL[row] = L;
+            a.sA[row] = make_uint2(acc, accv);
+        }
+    }
+
+    {
+        double vz[1] = {0.0};
+        uint32_t k4[4] = {n_tr, n_wall, n_core, n_terms};
+        seg_allreduce<T, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        if (leader) {
+            DevState &o = a.states[ic];
+            o.L = L;
+            o.U = U;
+            o.sweep = sweep;
+            o.cnt[0] += k4[0];
+            o.cnt[1] += acc_total;
+            o.cnt[2] += k4[1];
+            o.cnt[3] += k4[2];
+            o.cnt[4] += k4[3];
+            o.cnt[5] += n_vt;
+            o.cnt[6] += n_av;
+            o.cnt[7] += n_rc;
+            o.max_drift = mdrift;
+        }
+    }
+    double *xw = a.x + ch.x_off;
+    for (uint32_t t = 0; t < size; ++t) xw[start + t] = c.R[a0 + t];
+    __syncthreads();
+}
+
+// Persistent kernel, one instantiation per (precision, workgroup size,
+// memory mode): every workgroup pulls bundles from one queue (all potentials
+// together, longest first) until it is empty; all waves of a workgroup leave
+// together when the queue is drained.  The potential is uniform per bundle.
 template <typename Real, int T, bool GX>
-__global__ void __launch_bounds__(T) k_sweep_batched(SweepArgs a)
+__global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(JMM_WAVES_PER_EU)))
+k_sweep_batched(SweepArgs a)
 {
     extern __shared__ double smem[];
     __shared__ BatchedScratch<T> sc;
@@ -468,15 +824,25 @@ __global__ void __launch_bounds__(T) k_sweep_batched(SweepArgs a)
     for (;;) {
         if (threadIdx.x == 0) sc.item = atomicAdd(&a.ctrl->queue, 1u);
         __syncthreads();
-        const uint32_t item = sc.item;
+        // the same value in every lane: keep the bundle in scalar registers
+        const uint32_t item = __builtin_amdgcn_readfirstlane(sc.item);
         __syncthreads();
         if (item >= a.n_items) break;
         const DevBundle bd = a.bundles[item];
-        switch (bd.kind) {
-        case POT_LJ: run_bundle<Real, POT_LJ, T, GX>(a, bd, smem, sc, soff); break;
-        case POT_HARMONIC: run_bundle<Real, POT_HARMONIC, T, GX>(a, bd, smem, sc, soff); break;
-        case POT_HRSW: run_bundle<Real, POT_HRSW, T, GX>(a, bd, smem, sc, soff); break;
-        default: run_bundle<Real, POT_YUKAWA, T, GX>(a, bd, smem, sc, soff); break;
+        if (GX) {
+            switch (bd.kind) {
+            case POT_LJ: run_bundle_gx<Real, POT_LJ, T>(a, bd, sc, soff); break;
+            case POT_HARMONIC: run_bundle_gx<Real, POT_HARMONIC, T>(a, bd, sc, soff); break;
+            case POT_HRSW: run_bundle_gx<Real, POT_HRSW, T>(a, bd, sc, soff); break;
+            default: run_bundle_gx<Real, POT_YUKAWA, T>(a, bd, sc, soff); break;
+            }
+        } else {
+            switch (bd.kind) {
+            case POT_LJ: run_bundle_lds<Real, POT_LJ, T>(a, bd, smem, sc, soff); break;
+            case POT_HARMONIC: run_bundle_lds<Real, POT_HARMONIC, T>(a, bd, smem, sc, soff); break;
+            case POT_HRSW: run_bundle_lds<Real, POT_HRSW, T>(a, bd, smem, sc, soff); break;
+            default: run_bundle_lds<Real, POT_YUKAWA, T>(a, bd, smem, sc, soff); break;
+            }
         }
         __syncthreads();
     }
@@ -749,40 +1115,26 @@ hipError_t launch_begin(Ctrl *c, uint32_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-template <typename Real, int T, bool GX>
-static hipError_t launch_batched_t(const SweepArgs &a, uint32_t grid, size_t lds, hipStream_t s)
+template <typename Real>
+static const void *pick_shape(int T, int gx)
 {
-    hipLaunchKernelGGL((k_sweep_batched<Real, T, GX>), dim3(grid), dim3(T), lds, s, a);
-    return hipGetLastError();
+    if (T == 1024) return gx ? (const void *)k_sweep_batched<Real, 1024, true>
+                             : (const void *)k_sweep_batched<Real, 1024, false>;
+    return gx ? (const void *)k_sweep_batched<Real, 256, true>
+              : (const void *)k_sweep_batched<Real, 256, false>;
 }
 
 const void *batched_kernel(int fp32, int T, int gx)
 {
-    if (fp32) {
-        if (T == 1024) return gx ? (const void *)k_sweep_batched<float, 1024, true>
-                                 : (const void *)k_sweep_batched<float, 1024, false>;
-        return gx ? (const void *)k_sweep_batched<float, 256, true>
-                  : (const void *)k_sweep_batched<float, 256, false>;
-    }
-    if (T == 1024) return gx ? (const void *)k_sweep_batched<double, 1024, true>
-                             : (const void *)k_sweep_batched<double, 1024, false>;
-    return gx ? (const void *)k_sweep_batched<double, 256, true>
-              : (const void *)k_sweep_batched<double, 256, false>;
+    return fp32 ? pick_shape<float>(T, gx) : pick_shape<double>(T, gx);
 }
 
 hipError_t launch_batched(const SweepArgs &a, int fp32, int T, int gx, uint32_t grid, size_t lds,
                           hipStream_t s)
 {
-    if (fp32) {
-        if (T == 1024) return gx ? launch_batched_t<float, 1024, true>(a, grid, lds, s)
-                                 : launch_batched_t<float, 1024, false>(a, grid, lds, s);
-        return gx ? launch_batched_t<float, 256, true>(a, grid, lds, s)
-                  : launch_batched_t<float, 256, false>(a, grid, lds, s);
-    }
-    if (T == 1024) return gx ? launch_batched_t<double, 1024, true>(a, grid, lds, s)
-                             : launch_batched_t<double, 1024, false>(a, grid, lds, s);
-    return gx ? launch_batched_t<double, 256, true>(a, grid, lds, s)
-              : launch_batched_t<double, 256, false>(a, grid, lds, s);
+    const void *fn = batched_kernel(fp32, T, gx);
+    void *args[] = {(void *)&a};
+    return hipLaunchKernel(fn, dim3(grid), dim3(T), args, lds, s);
 }
 
 const void *plain_kernel(int gx)

@@ -273,6 +273,7 @@ void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w)
         if ((uint64_t)s->nbn < wc) wc = (uint64_t)s->nbn;
         if ((uint64_t)s->nbn + 1 < mc) mc = (uint64_t)s->nbn + 1;
     }
+    if (mc < 2) mc = 2; /* the hard-core test reads both nearest partners */
     if (N >= 1 && wc > N - 1) wc = N - 1;
     uint64_t nb = 1;
     if (mc >= 1 && mc <= N) {
@@ -401,14 +402,15 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
         if (nb_pos(c, (int64_t)i - 1, &xl) && (xn - xl) < c->gmin) { c->cnt[C_HC]++; return 0; }
         if (nb_pos(c, (int64_t)i + 1, &xr) && (xr - xn) < c->gmin) { c->cnt[C_HC]++; return 0; }
     }
-    double dE = 0.0;
+    /* left and right partner sums are kept apart and added at the end */
+    double dEL = 0.0, dER = 0.0;
     uint64_t terms = 0;
     for (uint64_t k = 1; k <= c->w; ++k) {
         double xj;
         if (!nb_pos(c, (int64_t)i - (int64_t)k, &xj)) break;
         double dold = xi - xj, dnew = xn - xj;
         if (dold > rc && dnew > rc) break;
-        dE += phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift);
+        dEL += phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift);
         terms += 2;
     }
     for (uint64_t k = 1; k <= c->w; ++k) {
@@ -416,9 +418,10 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
         if (!nb_pos(c, (int64_t)i + (int64_t)k, &xj)) break;
         double dold = xj - xi, dnew = xj - xn;
         if (dold > rc && dnew > rc) break;
-        dE += phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift);
+        dER += phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift);
         terms += 2;
     }
+    double dE = dEL + dER;
     c->cnt[C_TERMS] += terms;
     if (dE_out) *dE_out = dE;
     if (orc_accept(dE, c->beta, u)) {
