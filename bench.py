@@ -60,17 +60,12 @@ def algorithmic_flops(eng, chains, before):
     """sum over chains of f_pair*(n_old+n_new) + (n_old+n_new) + 8 per trial,
     with n_old+n_new the pair evaluations the kernel counted."""
     from jmmonedmc_amd.configs import flop_per_pair
-    tot = 0.0
-    terms_tot = 0
-    trials_tot = 0
-    for c, ch in enumerate(chains):
-        cnt = eng.counters(c)
-        terms = cnt["pair_terms"] - before[c]["pair_terms"]
-        trials = cnt["trials"] - before[c]["trials"]
-        tot += flop_per_pair(ch.potential) * terms + terms + 8.0 * trials
-        terms_tot += terms
-        trials_tot += trials
-    return tot, terms_tot, trials_tot
+    cnt = eng.all_counters()
+    terms = (cnt["pair_terms"] - before["pair_terms"]).astype(np.float64)
+    trials = (cnt["trials"] - before["trials"]).astype(np.float64)
+    fp = np.array([flop_per_pair(ch.potential) for ch in chains], dtype=np.float64)
+    tot = float((fp * terms + terms + 8.0 * trials).sum())
+    return tot, int(terms.sum()), int(trials.sum())
 
 
 def block_sem(x, nblocks=20):
@@ -90,7 +85,7 @@ def run_form(chains, form, steps, warmup, bs, graph, label):
     eng.sync()
     eng.read_samples(cap)
     eng.kernel_time(reset=True)
-    before = [eng.counters(c) for c in range(len(chains))]
+    before = eng.all_counters()
     print(f"[bench] {label}: timing {steps} blocks of {bs} sweeps", file=sys.stderr, flush=True)
     eng.sync()
     t0 = time.perf_counter()

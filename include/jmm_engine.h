@@ -166,6 +166,11 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
  * counter.  Any output pointer may be NULL.  Synchronous. */
 int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uint64_t *sweep);
 
+/* Bulk form of jmm_get_state for every chain: x[sum N] with chain c at
+ * offset sum_{c'<c} N_c' (chain order of creation), L[n_chains], U[n_chains].
+ * Any output may be NULL.  Synchronous. */
+int jmm_get_states(jmm_engine *e, double *x, double *L, double *U);
+
 /* Restart: installs positions (must be sorted and respect the hard core),
  * length and sweep counter; the running energy is recomputed in FP64.
  * Synchronous. */
@@ -175,7 +180,8 @@ int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t 
  * energy (the next recompute must detect and repair it).  Synchronous. */
 int jmm_perturb_energy(jmm_engine *e, int chain, double delta);
 
-/* Replaces printAcc/printE (src/jmmMCState.cpp:2143-2156). Synchronous. */
+/* Replaces printAcc/printE (src/jmmMCState.cpp:2143-2156). chain = -1
+ * fills out[0..n_chains) for every chain in one copy. Synchronous. */
 int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out);
 
 /* Accept log (opts.log_accepts): out[c][s][i] one byte per displacement
@@ -205,6 +211,9 @@ int jmm_observables(jmm_engine *e, double *obs, double *grad);
  * x_dev[n_configs*N] (sorted per configuration), L_dev[n_configs];
  * obs_dev[n_configs*6] as in jmm_observables; grad_dev (nullable)
  * [n_configs*N] = dU/dx.  precision: 32 or 64.  stream: hipStream_t or NULL.
+ * Every gap must be at least the potential's hard-core distance (0.5 sigma
+ * for LJ, p2 for Yukawa, p0 for hard rods), as in any state a chain visits:
+ * partners are searched within floor((cutoff + max_step)/hard core) indices.
  * Asynchronous on that stream.  No reference counterpart. */
 int jmm_pair_observables(const jmm_chain_spec *spec, int n_configs, const double *x_dev,
                          const double *L_dev, double *obs_dev, double *grad_dev, int precision,

@@ -21,7 +21,7 @@ FORMS = {"plain_fp64": FORM_PLAIN_FP64, "batched_fp64": FORM_BATCHED_FP64,
 EXPORTED = [
     "jmm_abi_version", "jmm_last_error", "jmm_schedule", "jmm_create", "jmm_destroy",
     "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples",
-    "jmm_get_state", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
+    "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
 ]
 
@@ -86,6 +86,7 @@ def lib(path: str = LIB):
     L.jmm_full_energy.argtypes = [vp, dp]
     L.jmm_read_samples.argtypes = [vp, C.POINTER(Sample), C.c_uint32, u32p]
     L.jmm_get_state.argtypes = [vp, C.c_int, dp, dp, dp, u64p]
+    L.jmm_get_states.argtypes = [vp, dp, dp, dp]
     L.jmm_set_state.argtypes = [vp, C.c_int, dp, C.c_double, C.c_uint64]
     L.jmm_perturb_energy.argtypes = [vp, C.c_int, C.c_double]
     L.jmm_get_counters.argtypes = [vp, C.c_int, C.POINTER(Counters)]

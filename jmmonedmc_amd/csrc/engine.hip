@@ -736,6 +736,30 @@ int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uin
     return 0;
 }
 
+int jmm_get_states(jmm_engine *e, double *x, double *L, double *U)
+{
+    if (!e) return fail(JMM_EINVAL, "jmm_get_states: null engine");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (x) {
+        std::vector<double> hx(e->total_N);
+        HIPCHK(hipMemcpy(hx.data(), e->d_x, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < e->n; ++k) {
+            const uint32_t c = e->ext_of_int[k];
+            std::memcpy(x + e->ext_off[c], hx.data() + e->hch[k].x_off, sizeof(double) * e->hch[k].N);
+        }
+    }
+    if (L || U) {
+        std::vector<DevState> st(e->n);
+        HIPCHK(hipMemcpy(st.data(), e->d_states, sizeof(DevState) * e->n, hipMemcpyDeviceToHost));
+        for (int k = 0; k < e->n; ++k) {
+            const uint32_t c = e->ext_of_int[k];
+            if (L) L[c] = st[k].L;
+            if (U) U[c] = st[k].U;
+        }
+    }
+    return 0;
+}
+
 int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t sweep)
 {
     if (!e || chain < 0 || chain >= e->n || !x) return fail(JMM_EINVAL, "jmm_set_state: bad argument");
@@ -775,13 +799,8 @@ int jmm_perturb_energy(jmm_engine *e, int chain, double delta)
     return 0;
 }
 
-int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
+static void fill_counters(const DevState &st, jmm_counters *out)
 {
-    if (!e || chain < 0 || chain >= e->n || !out) return fail(JMM_EINVAL, "jmm_get_counters: bad argument");
-    HIPCHK(hipStreamSynchronize(e->stream));
-    const uint32_t k = e->int_of_ext[chain];
-    DevState st;
-    HIPCHK(hipMemcpy(&st, e->d_states + k, sizeof st, hipMemcpyDeviceToHost));
     out->sweeps = st.sweep;
     out->trials = st.cnt[0];
     out->acc_d = st.cnt[1];
@@ -794,6 +813,21 @@ int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
     out->max_drift = st.max_drift;
     out->U = st.U;
     out->L = st.L;
+}
+
+int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
+{
+    if (!e || chain < -1 || chain >= e->n || !out) return fail(JMM_EINVAL, "jmm_get_counters: bad argument");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (chain >= 0) {
+        DevState st;
+        HIPCHK(hipMemcpy(&st, e->d_states + e->int_of_ext[chain], sizeof st, hipMemcpyDeviceToHost));
+        fill_counters(st, out);
+        return 0;
+    }
+    std::vector<DevState> st(e->n);
+    HIPCHK(hipMemcpy(st.data(), e->d_states, sizeof(DevState) * e->n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < e->n; ++k) fill_counters(st[k], out + e->ext_of_int[k]);
     return 0;
 }
 

@@ -142,6 +142,14 @@ class Engine:
               "jmm_get_state")
         return x, L.value, U.value, sw.value
 
+    def states(self):
+        """All chains at once: (list of position arrays, L[n], U[n])."""
+        x = np.zeros(int(self.Ns.sum()))
+        L = np.zeros(self.n)
+        U = np.zeros(self.n)
+        check(lib().jmm_get_states(self.h, _dp(x), _dp(L), _dp(U)), "jmm_get_states")
+        return [x[o: o + n] for o, n in zip(self.offsets, self.Ns)], L, U
+
     def set_state(self, chain: int, x, L: float, sweep: int = 0):
         x = np.ascontiguousarray(x, dtype=np.float64)
         check(lib().jmm_set_state(self.h, chain, _dp(x), L, sweep), "jmm_set_state")
@@ -153,6 +161,14 @@ class Engine:
         c = Counters()
         check(lib().jmm_get_counters(self.h, chain, C.byref(c)), "jmm_get_counters")
         return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+    def all_counters(self) -> dict:
+        """Every chain's counters in one call: dict of arrays indexed by chain."""
+        buf = (Counters * self.n)()
+        check(lib().jmm_get_counters(self.h, -1, buf), "jmm_get_counters")
+        raw = np.frombuffer(buf, dtype=[(k, "u8" if t is C.c_uint64 else "f8")
+                                        for k, t in Counters._fields_])
+        return {k: raw[k].copy() for k, _ in Counters._fields_}
 
     def full_energy(self):
         U = np.zeros(self.n)

@@ -42,7 +42,9 @@ def gap_moments(phi, beta, P, gmin, gmax=math.inf, breaks=()):
     m2 = integ(lambda g: g * g) / Z
     ephi = integ(phi) / Z
     ephig = integ(lambda g: phi(g) * g) / Z
-    return dict(m=m, v=m2 - m * m, ephi=ephi, ephig=ephig)
+    ephi2 = integ(lambda g: phi(g) * phi(g)) / Z
+    return dict(m=m, v=m2 - m * m, ephi=ephi, ephig=ephig, ephi2=ephi2,
+                lnZ=math.log(Z) - ref)
 
 
 def walls_npt(N, mom, beta, P):

@@ -1,0 +1,131 @@
+"""Engine services on the GPU, through the C ABI: restart from a saved state
+(printCoords/restart), the drift guard (ECheck) with an injected energy
+error, the sample buffer's full condition, graph replay against eager
+launches, and the bulk counters against the per-chain ones."""
+import numpy as np
+import pytest
+
+from jmmonedmc_amd._native import JmmError
+from jmmonedmc_amd.engine import Chain, Engine
+
+from helpers import CASES, lj, oracle_spec, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64"])
+def test_restart_from_saved_state_replays_oracle(gpu, oracle, form):
+    """Run 12 sweeps, save, restore into a fresh engine at sweep 12 and run 20
+    more: positions and decisions equal the oracle's uninterrupted run."""
+    print(f"[gpu] restart {form}", flush=True)
+    chains = CASES["npt_periodic_log"]()
+    with Engine(chains, form=form, sample_capacity=64, block_sweeps=8) as a:
+        a.run(12)
+        saved = [a.state(c) for c in range(len(chains))]
+    with Engine(chains, form=form, sample_capacity=64, block_sweeps=8, log_accepts=True) as b:
+        for c, (x, L, U, sw) in enumerate(saved):
+            assert sw == 12
+            b.set_state(c, x, L, sw)
+        b.run(20)
+        logs = b.read_accept_log(64)
+        for c, ch in enumerate(chains):
+            orc = oracle.Chain(oracle_spec(oracle, ch))
+            orc.sweeps(12)
+            _, olog = orc.sweeps(20, log=True)
+            x, L, U, sw = b.state(c)
+            ox, oL, oU, osw = orc.state()
+            assert sw == osw == 32
+            assert np.array_equal(logs[c].ravel(), olog)
+            assert np.array_equal(x, ox) and L == oL
+            assert rel(U, oU) <= 1e-10
+
+
+def test_set_state_rejects_invalid_configurations(gpu):
+    ch = Chain(N=16, L0=32.0, seed=1, **lj())
+    with Engine([ch], form="batched_fp64") as e:
+        x, L, _, _ = e.state(0)
+        bad = x.copy()
+        bad[3], bad[4] = bad[4], bad[3]
+        with pytest.raises(JmmError, match="unsorted or inside the hard core"):
+            e.set_state(0, bad, L)
+        with pytest.raises(JmmError, match="bad length"):
+            e.set_state(0, x, 6.0)                    # periodic length must exceed 2 (rc + step)
+        e.set_state(0, x + 0.25, L, 5)
+        x2, L2, U2, sw2 = e.state(0)
+        assert np.array_equal(x2, x + 0.25) and sw2 == 5
+
+
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64", "batched_fp32"])
+def test_drift_guard_repairs_injected_energy_error(gpu, form):
+    """eci = 10: an error of +0.5 injected into chain 1's running energy after
+    sweep 10 is found by the recompute at sweep 20 and replaced by the FP64
+    energy; the other chains see only rounding-level drift."""
+    print(f"[gpu] drift guard {form}", flush=True)
+    chains = [Chain(N=96, rho=0.5, T=1.0, seed=2, chain_id=c, eci=10, **lj()) for c in range(3)]
+    with Engine(chains, form=form, sample_capacity=64, block_sweeps=10) as e:
+        e.run(10)
+        e.sync()
+        e.perturb_energy(1, 0.5)
+        e.run(10)
+        smp = e.read_samples(64)
+        cnt = e.all_counters()
+        U = e.full_energy()
+        assert cnt["recomputes"].tolist() == [2, 2, 2]
+        assert 0.5 - 1e-4 <= cnt["max_drift"][1] <= 0.5 + 1e-4
+        tol = 1e-9 if form != "batched_fp32" else 1e-4
+        assert cnt["max_drift"][0] <= tol and cnt["max_drift"][2] <= tol
+        for c in range(3):
+            assert rel(cnt["U"][c], U[c]) <= 1e-12
+        # sweeps 11..19 carry the injected offset; sweep 20 is the repaired value
+        assert abs(smp["U"][1, 19] - U[1]) <= 1e-9 * max(1.0, abs(U[1]))
+
+
+def test_sample_buffer_full_and_drain(gpu):
+    ch = Chain(N=32, L0=64.0, seed=4, **lj())
+    with Engine([ch], form="batched_fp64", sample_capacity=20, block_sweeps=10) as e:
+        e.run(20)
+        with pytest.raises(JmmError, match="read samples first"):
+            e.run(1)
+        s = e.read_samples(20)
+        assert s.shape == (1, 20)
+        e.run(5)
+        s2 = e.read_samples(20)
+        assert s2.shape == (1, 5)
+        assert e.counters(0)["sweeps"] == 25
+
+
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64", "batched_fp32"])
+def test_graph_replay_equals_eager(gpu, form):
+    """use_graph replays the captured sweep block: after four blocks with a
+    sample read and host synchronisation between them, every replay's
+    samples and the final state equal the eager engine's bit for bit."""
+    print(f"[gpu] graph replay {form}", flush=True)
+    chains = CASES["mixed_batch"]()
+    out = []
+    for graph in (False, True):
+        with Engine(chains, form=form, sample_capacity=16, block_sweeps=16, use_graph=graph) as e:
+            rows = []
+            for _ in range(4):
+                e.run(16)
+                e.sync()
+                rows.append(e.read_samples(16))
+            xs, Ls, Us = e.states()
+            out.append((rows, np.concatenate(xs), Ls, Us))
+    (ra, xa, La, Ua), (rb, xb, Lb, Ub) = out
+    for a, b in zip(ra, rb):
+        assert np.array_equal(a, b)
+    assert np.array_equal(xa, xb) and np.array_equal(La, Lb) and np.array_equal(Ua, Ub)
+
+
+def test_bulk_counters_equal_per_chain(gpu):
+    chains = CASES["mixed_batch"]()
+    with Engine(chains, form="batched_fp32", sample_capacity=32) as e:
+        e.run(24)
+        allc = e.all_counters()
+        xs, Ls, Us = e.states()
+        for c in range(len(chains)):
+            one = e.counters(c)
+            for k, v in one.items():
+                assert allc[k][c] == v, (c, k)
+            x, L, U, _ = e.state(c)
+            assert np.array_equal(xs[c], x) and Ls[c] == L and Us[c] == U

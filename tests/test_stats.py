@@ -1,0 +1,156 @@
+"""Statistics layer (jmmonedmc_amd/stats.py), pinned without retyping its
+formulas: hand-computed block tables, AR(1) closed forms, and the
+fluctuation formulas against thermodynamic derivatives of the exact 1D
+NPT partition function (finite differences of quadratures)."""
+import math
+
+import numpy as np
+import pytest
+
+from jmmonedmc_amd import stats
+import exact1d
+
+
+def test_block_sem_hand_table():
+    x = np.arange(1.0, 10.0)                  # 9 samples; block 2 drops the 9th
+    assert np.allclose(stats.block_means(x, 2), [1.5, 3.5, 5.5, 7.5])
+    # means 1.5..7.5: population std sqrt(5), 4 blocks -> sqrt(5)/2
+    assert stats.block_sem(x, 2) == pytest.approx(math.sqrt(5.0) / 2.0, rel=1e-15)
+    tab = stats.block_sem_scan(x, 2)
+    assert tab[:, 0].tolist() == [2.0, 4.0] and tab[:, 1].tolist() == [4.0, 2.0]
+    # block 4: means 2.5, 6.5 -> std 2, sem 2/sqrt(2)
+    assert tab[1, 2] == pytest.approx(math.sqrt(2.0), rel=1e-15)
+    assert math.isnan(stats.block_sem(x, 9))
+
+
+def _ar1(a, n, seed):
+    rng = np.random.default_rng(seed)
+    e = rng.standard_normal(n)
+    x = np.empty(n)
+    x[0] = e[0] / math.sqrt(1 - a * a)
+    for t in range(1, n):
+        x[t] = a * x[t - 1] + e[t]
+    return x
+
+
+def test_ar1_closed_forms():
+    """AR(1) x_t = a x_{t-1} + e_t: rho(t) = a^t, tau_int = (1+a)/(2(1-a)),
+    adjacent-sample correlation = a, and the large-block SEM approaches
+    sqrt(2 tau Var x / n)."""
+    a, n = 0.8, 400_000
+    x = _ar1(a, n, 7)
+    rho = stats.autocorrelation(x, 5)
+    assert np.allclose(rho, a ** np.arange(6), atol=0.01)
+    assert stats.lag_correlation(x, 3) == pytest.approx(a ** np.arange(4), abs=0.01)
+    assert stats.block_adjacent_correlation(x, 2)[1] == pytest.approx(a, abs=0.01)
+    tau = stats.tau_int(x)
+    assert tau == pytest.approx((1 + a) / (2 * (1 - a)), rel=0.05)
+    want = math.sqrt(2 * tau * x.var() / n)
+    assert stats.block_sem(x, 2000) == pytest.approx(want, rel=0.2)
+    iid = np.random.default_rng(3).standard_normal(n)
+    assert abs(stats.tau_int(iid) - 0.5) < 0.05
+    assert abs(stats.block_adjacent_correlation(iid, 4)[4]) < 0.02
+
+
+def _walls_moments(phi, T, P, N, gmin):
+    """Exact NPT moments of the walled 1D nearest-neighbour chain: two wall
+    gaps of law exp(-P g/T), N-1 interior gaps of law exp(-(phi+P g)/T)."""
+    mom = exact1d.gap_moments(phi, 1.0 / T, P, gmin)
+    a = T / P                                  # wall gap mean; its variance is a^2
+    L = 2 * a + (N - 1) * mom["m"]
+    E = (N - 1) * mom["ephi"]
+    varL = 2 * a * a + (N - 1) * mom["v"]
+    cov = (N - 1) * (mom["ephig"] - mom["m"] * mom["ephi"])
+    varE = (N - 1) * (mom["ephi2"] - mom["ephi"] ** 2)
+    lnD = 2 * math.log(a) + (N - 1) * mom["lnZ"]
+    return dict(L=L, E=E, L2=varL + L * L, E2=varE + E * E, LE=cov + L * E, lnD=lnD)
+
+
+@pytest.mark.parametrize("name,phi,gmin", [
+    ("harmonic", exact1d.harmonic_phi(), 1e-12),
+    ("lj", exact1d.lj_phi(), 0.5),
+])
+def test_fluctuation_formulas_match_thermodynamic_derivatives(name, phi, gmin):
+    """cP = d<H>/dT|P, betaT = -(1/L) dL/dP|T, alphaP = (1/L) dL/dT|P,
+    gammaV = -(dL/dT)/(dL/dP), muJT = -(dH/dP|T)/(dH/dT|P) and
+    betaS = -(1/L)(dL/dP|T - dL/dT|P dS/dP|T / dS/dT|P), with H = E + P L and
+    S = (<H> - G)/T, G = -T ln Delta(T, P), all by central differences."""
+    N, T, P = 7, 0.9, 0.6
+    m = _walls_moments(phi, T, P, N, gmin)
+    f = stats.fluctuation_properties(m["E"], m["E2"], m["L"], m["L2"], m["LE"], P, T)
+
+    def H(T_, P_):
+        q = _walls_moments(phi, T_, P_, N, gmin)
+        return q["E"] + P_ * q["L"]
+
+    def S(T_, P_):
+        q = _walls_moments(phi, T_, P_, N, gmin)
+        return (q["E"] + P_ * q["L"] + T_ * q["lnD"]) / T_
+
+    def Lof(T_, P_):
+        return _walls_moments(phi, T_, P_, N, gmin)["L"]
+
+    h = 1e-4
+    dLdT = (Lof(T + h, P) - Lof(T - h, P)) / (2 * h)
+    dLdP = (Lof(T, P + h) - Lof(T, P - h)) / (2 * h)
+    dHdT = (H(T + h, P) - H(T - h, P)) / (2 * h)
+    dHdP = (H(T, P + h) - H(T, P - h)) / (2 * h)
+    dSdT = (S(T + h, P) - S(T - h, P)) / (2 * h)
+    dSdP = (S(T, P + h) - S(T, P - h)) / (2 * h)
+    L = m["L"]
+    want = dict(cP=dHdT, betaT=-dLdP / L, alphaP=dLdT / L, gammaV=-dLdT / dLdP,
+                muJT=-dHdP / dHdT, betaS=-(dLdP - dLdT * dSdP / dSdT) / L)
+    for k, v in want.items():
+        assert float(f[k]) == pytest.approx(v, rel=1e-6), (name, k, float(f[k]), v)
+    # Maxwell relation dS/dP|T = -dL/dT|P holds for the quadrature itself
+    assert dSdP == pytest.approx(-dLdT, rel=1e-6)
+
+
+def test_fluctuations_from_samples_ideal_gas():
+    """Walled ideal gas (no interactions) at NPT: L is Gamma(N+1, T/P), E = 0.
+    Then cP = N+1, betaT = 1/P, alphaP = 1/T (exact), from per-sweep samples."""
+    N, T, P = 20, 1.3, 0.7
+    rng = np.random.default_rng(11)
+    L = rng.gamma(N + 1, T / P, size=200_000)
+    U = np.zeros_like(L)
+    out = stats.fluctuations_from_samples(U, L, P, T, nblocks=20)
+    for k, v in dict(cP=N + 1.0, betaT=1 / P, alphaP=1 / T).items():
+        mean, sem = out[k]
+        assert abs(mean - v) < 4 * sem + 1e-3 * abs(v), (k, mean, sem, v)
+
+
+def test_fluctuation_gradient_equals_reweighted_derivative(oracle):
+    """d<A>/dtheta from the fluctuation formula equals the theta-derivative of
+    the reweighted average sum A_theta' e^{-beta (U_theta' - U_theta)} / sum
+    e^{...} on the same samples (central differences), for A = W."""
+    import dataclasses
+    from jmmonedmc_amd.autograd import fluctuation_gradient
+    from jmmonedmc_amd.engine import Chain
+    from helpers import lj, oracle_spec
+    N, L0, beta = 32, 48.0, 0.8
+    rng = np.random.default_rng(2)
+    X = []
+    for _ in range(40):
+        sp = np.diff(np.concatenate([[0.0], np.sort(rng.random(N - 1)), [1.0]]))
+        g = 0.85 + (L0 - 0.85 * N) * sp
+        X.append(np.cumsum(g) - g[0] - L0 / 2 + 0.5 * g[0])
+    base = Chain(N=N, L0=L0, T=1 / beta, **lj())
+
+    def obs(theta):
+        ch = dataclasses.replace(base, pot_p=(theta[0], theta[1], 0.0, 0.0))
+        s = oracle_spec(oracle, ch)
+        return np.array([oracle.observables_x(s, x, L0, want_grad=False)[0] for x in X])
+
+    o0 = obs([1.0, 1.0])
+    fg = fluctuation_gradient(o0[:, 1], o0[:, 4:6], o0[:, 2:4], beta)
+    h = 1e-5
+    for k in range(2):
+        v = []
+        for sg in (1, -1):
+            th = [1.0, 1.0]
+            th[k] += sg * h
+            o = obs(th)
+            wgt = np.exp(-beta * (o[:, 0] - o0[:, 0]))
+            v.append((o[:, 1] * wgt).sum() / wgt.sum())
+        fd = (v[0] - v[1]) / (2 * h)
+        assert fg[k] == pytest.approx(fd, rel=1e-6, abs=1e-8), (k, fg[k], fd)
