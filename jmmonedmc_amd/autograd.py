@@ -49,6 +49,7 @@ class PairEnergy(torch.autograd.Function):
         obs, grad = _observables(x, L, chain, theta.detach().cpu().numpy(), precision,
                                  want_grad=True)
         ctx.save_for_backward(grad, obs[:, 2:4].clone())
+        ctx.theta_device = theta.device
         return obs[:, 0].clone()
 
     @staticmethod
@@ -56,7 +57,7 @@ class PairEnergy(torch.autograd.Function):
         grad, dUdth = ctx.saved_tensors
         gx = gU[:, None] * grad
         gth = (gU[:, None] * dUdth).sum(0)
-        return gx, None, gth.to(dtype=torch.float64), None, None
+        return gx, None, gth.to(device=ctx.theta_device, dtype=torch.float64), None, None
 
 
 def observables(x, L, chain: Chain, theta, precision: int = 64):
