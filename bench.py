@@ -95,11 +95,13 @@ def run_form(chains, form, steps, warmup, bs, graph, label):
     t1 = time.perf_counter()
     launches, ms, sweeps = eng.kernel_time(reset=True)
     flops, terms, trials = algorithmic_flops(eng, chains, before)
+    t2 = time.perf_counter()
     smp = eng.read_samples(cap)
+    readback = time.perf_counter() - t2
     eng.close()
     wall = t1 - t0
     return dict(wall_s=wall, trials=trials, terms=terms, flops=flops, launches=launches,
-                kernel_ms=ms, sweeps=sweeps, samples=smp,
+                kernel_ms=ms, sweeps=sweeps, samples=smp, readback_s=readback,
                 trials_per_s=trials / wall, ms_per_step=1e3 * wall / steps)
 
 
@@ -223,6 +225,9 @@ def main():
                    "sweeps_timed": a.steps * a.block_sweeps, "block_sweeps": a.block_sweeps,
                    "form": "batched_fp32", "parallelism": "1 GPU, chains packed into workgroups"},
         "chain_sweeps_per_s": len(chains) * a.steps * a.block_sweeps / fp32["wall_s"],
+        "sample_readback": {"seconds": fp32["readback_s"],
+                            "bytes": int(fp32["samples"].size * 24),
+                            "value_including_readback": fp32["trials"] / (fp32["wall_s"] + fp32["readback_s"])},
         "plain_fp64": plain and {"value": plain["trials_per_s"], "sweeps_timed": plain["sweeps"],
                                  "ms_per_block": plain["ms_per_step"]},
         "speedup_vs_plain_fp64": plain and fp32["trials_per_s"] / plain["trials_per_s"],

@@ -67,7 +67,7 @@ def build_driver(verbose: bool = False) -> str:
     if not srcs:
         return ""
     tmp = DRIVER + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, *srcs,
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fopenmp", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, *srcs,
           f"-L{OUT}", "-ljmm_engine", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, DRIVER)
     return DRIVER

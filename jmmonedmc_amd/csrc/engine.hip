@@ -415,6 +415,7 @@ int jmm_schedule(const jmm_chain_spec *spec, uint64_t *B, uint64_t *M_min, uint6
 int jmm_destroy(jmm_engine *e)
 {
     if (!e) return 0;
+    (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -646,6 +647,7 @@ int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps)
 int jmm_synchronize(jmm_engine *e)
 {
     if (!e) return fail(JMM_EINVAL, "jmm_synchronize: null engine");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -653,6 +655,7 @@ int jmm_synchronize(jmm_engine *e)
 int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *sweeps, int reset)
 {
     if (!e) return fail(JMM_EINVAL, "jmm_kernel_time: null engine");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     double tot = 0.0;
     for (auto &p : e->ev) {
@@ -677,6 +680,7 @@ int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *swe
 int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out)
 {
     if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_samples: null argument");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     const uint32_t n = e->write_pos;
     if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_samples: %u sweeps pending, room for %u", n, max_sweeps);
@@ -709,6 +713,7 @@ int jmm_read_accept_log(jmm_engine *e, uint8_t *out, uint32_t max_sweeps, uint32
 {
     if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_accept_log: null argument");
     if (!e->d_log) return fail(JMM_ESTATE, "engine was created without log_accepts");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     const uint32_t n = e->write_pos;
     if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_accept_log: %u sweeps pending, room for %u", n, max_sweeps);
@@ -725,6 +730,7 @@ int jmm_read_accept_log(jmm_engine *e, uint8_t *out, uint32_t max_sweeps, uint32
 int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uint64_t *sweep)
 {
     if (!e || chain < 0 || chain >= e->n) return fail(JMM_EINVAL, "jmm_get_state: bad chain");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     const uint32_t k = e->int_of_ext[chain];
     if (x) HIPCHK(hipMemcpy(x, e->d_x + e->hch[k].x_off, sizeof(double) * e->hch[k].N, hipMemcpyDeviceToHost));
@@ -739,6 +745,7 @@ int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uin
 int jmm_get_states(jmm_engine *e, double *x, double *L, double *U)
 {
     if (!e) return fail(JMM_EINVAL, "jmm_get_states: null engine");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (x) {
         std::vector<double> hx(e->total_N);
@@ -763,6 +770,7 @@ int jmm_get_states(jmm_engine *e, double *x, double *L, double *U)
 int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t sweep)
 {
     if (!e || chain < 0 || chain >= e->n || !x) return fail(JMM_EINVAL, "jmm_set_state: bad argument");
+    HIPCHK(hipSetDevice(e->device));
     const jmm_chain_spec &s = e->specs[chain];
     const uint32_t k = e->int_of_ext[chain];
     const DevChain &d = e->hch[k];
@@ -790,6 +798,7 @@ int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t 
 int jmm_perturb_energy(jmm_engine *e, int chain, double delta)
 {
     if (!e || chain < 0 || chain >= e->n) return fail(JMM_EINVAL, "jmm_perturb_energy: bad chain");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     const uint32_t k = e->int_of_ext[chain];
     DevState st;
@@ -818,6 +827,7 @@ static void fill_counters(const DevState &st, jmm_counters *out)
 int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
 {
     if (!e || chain < -1 || chain >= e->n || !out) return fail(JMM_EINVAL, "jmm_get_counters: bad argument");
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (chain >= 0) {
         DevState st;
@@ -876,12 +886,14 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32)
 int jmm_observables(jmm_engine *e, double *obs, double *grad)
 {
     if (!e || !obs) return fail(JMM_EINVAL, "jmm_observables: null argument");
+    HIPCHK(hipSetDevice(e->device));
     return observables_impl(e, obs, grad, e->form == JMM_FORM_BATCHED_FP32);
 }
 
 int jmm_full_energy(jmm_engine *e, double *U_out)
 {
     if (!e || !U_out) return fail(JMM_EINVAL, "jmm_full_energy: null argument");
+    HIPCHK(hipSetDevice(e->device));
     std::vector<double> obs((size_t)e->n * 6);
     if (int rc = observables_impl(e, obs.data(), nullptr, 0)) return rc;
     for (int c = 0; c < e->n; ++c) U_out[c] = obs[(size_t)c * 6];

@@ -3,6 +3,8 @@
 // while (incrementStep) { Step; interval-gated output }, then the final
 // energy and acceptance counts.  The chain work runs on the GPU through the
 // jmmMCState-style facade; the file name of INPUT may be given as argv[1].
+#include <omp.h>
+
 #include <cstdio>
 #include <string>
 
@@ -30,33 +32,55 @@ int main(int argc, char *argv[])
     struct MCState *mcs = setupMCS(inp);
     if (!mcs) return 1;
     std::printf("Setup completed\n");
-    printStep(mcs);
 
-    unsigned long nnn = 0;
-    double ddd = 0.5;
-    fad(mcs, &nnn, &ddd);
-    if (getRelaxFlag(mcs) > 0) relaxVolume(mcs);
-
-    if (!getRestartFlag(mcs)) {
-        printCoords(mcs);
-        printRho(mcs);
-        updateThermo(mcs);
-        printThermo(mcs);
-        printG(mcs);
-    }
-
-    while (incrementStep(mcs)) {
-        if (Step(mcs)) {
-            std::fprintf(stderr, "Step failed\n");
-            freeMCS(mcs);
-            return 1;
+    // the reference's team structure: collective calls (fad, relaxVolume,
+    // printG, incrementStep, Step) by every thread, output by one
+    int failed = 0;
+#pragma omp parallel default(shared)
+    {
+#pragma omp single
+        {
+            std::printf("Running with %d threads...\n\n", omp_get_num_threads());
+            printStep(mcs);
         }
-        if (isCoordPrint(mcs)) printCoords(mcs);
-        if (isThermoPrint(mcs)) printThermo(mcs);
-        if (isRhoPrint(mcs)) printRho(mcs);
-        if (isMaxDisAdjust(mcs)) maxDisAdjust(mcs);
-        if (isMaxDVAdjust(mcs)) maxDVAdjust(mcs);
-        if (isGPrint(mcs)) printG(mcs);
+        unsigned long nnn = 0;
+        double ddd = 0.5;
+        fad(mcs, &nnn, &ddd);
+        if (getRelaxFlag(mcs) > 0) relaxVolume(mcs);
+
+        if (!getRestartFlag(mcs)) {
+#pragma omp single
+            {
+                printCoords(mcs);
+                printRho(mcs);
+                updateThermo(mcs);
+                printThermo(mcs);
+            }
+            printG(mcs);
+        }
+
+        while (incrementStep(mcs)) {
+            if (Step(mcs)) {
+#pragma omp single
+                failed = 1;
+                break;
+            }
+#pragma omp single
+            {
+                if (isCoordPrint(mcs)) printCoords(mcs);
+                if (isThermoPrint(mcs)) printThermo(mcs);
+                if (isRhoPrint(mcs)) printRho(mcs);
+                if (isMaxDisAdjust(mcs)) maxDisAdjust(mcs);
+                if (isMaxDVAdjust(mcs)) maxDVAdjust(mcs);
+            }
+            if (isGPrint(mcs)) printG(mcs);
+            std::fflush(stdout);
+        }
+    }
+    if (failed) {
+        std::fprintf(stderr, "Step failed\n");
+        freeMCS(mcs);
+        return 1;
     }
 
     std::printf("\nPROGRAM COMPLETED SUCCESSFULLY!\n");

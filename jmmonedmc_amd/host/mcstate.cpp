@@ -17,6 +17,23 @@
 
 #include "jmm_engine.h"
 
+// The reference's fad, incrementStep, Step, printG and relaxVolume are
+// collective: src/Main.cpp:50-178 calls them from every thread of an OpenMP
+// team and they share the work through orphaned single/barrier constructs.
+// Here one thread of the team does the work and every thread returns its
+// result; outside a parallel region (or with a team of one) they are plain
+// calls.
+template <class R, class F>
+static R collective(F f)
+{
+    static R shared;
+#pragma omp single
+    shared = f();
+    const R r = shared;
+#pragma omp barrier
+    return r;
+}
+
 struct MCState {
     MCInput inp;
     jmm_engine *eng = nullptr;
@@ -275,29 +292,35 @@ int printMCP(struct MCState *m)
 int fad(struct MCState *m, unsigned long *, double *)
 {
     // the step-0 full energy (src/Main.cpp:66-68) is computed in FP64 at setup
-    jmm_counters c{};
-    int rc = jmm_get_counters(m->eng, 0, &c);
-    m->E = c.U;
-    return rc;
+    return collective<int>([m] {
+        jmm_counters c{};
+        int rc = jmm_get_counters(m->eng, 0, &c);
+        m->E = c.U;
+        return rc;
+    });
 }
 
 unsigned long incrementStep(struct MCState *m)
 {
-    if (m->sn >= m->numSteps) return 0;
-    m->lastStep = m->sn;
-    m->sn += m->numTrialTypes;
-    if (m->sn / 10000 != m->lastStep / 10000) printStep(m);
-    return m->sn;
+    return collective<unsigned long>([m]() -> unsigned long {
+        if (m->sn >= m->numSteps) return 0;
+        m->lastStep = m->sn;
+        m->sn += m->numTrialTypes;
+        if (m->sn / 10000 != m->lastStep / 10000) printStep(m);
+        return m->sn;
+    });
 }
 
 int Step(struct MCState *m)
 {
-    m->pending++;
-    if (m->pending >= m->cap) {
-        if (int rc = flush(m)) return rc;
-    }
-    if (isECheck(m)) ECheck(m);
-    return 0;
+    return collective<int>([m] {
+        m->pending++;
+        if (m->pending >= m->cap) {
+            if (int rc = flush(m)) return rc;
+        }
+        if (isECheck(m)) ECheck(m);
+        return 0;
+    });
 }
 
 int isCoordPrint(struct MCState *m) { return crossed(m, m->inp.cpi); }
@@ -361,7 +384,7 @@ int printCoords(struct MCState *m)
 }
 
 int printRho(struct MCState *) { return 0; }
-int printG(struct MCState *) { return 0; }
+int printG(struct MCState *) { return collective<int>([] { return 0; }); }
 
 int ECheck(struct MCState *m)
 {
@@ -393,8 +416,10 @@ int maxDVAdjust(struct MCState *) { return 0; }
 
 int relaxVolume(struct MCState *)
 {
-    std::printf("NOTE: RELAX requested; volume relaxation is not part of this engine, skipped.\n");
-    return 0;
+    return collective<int>([] {
+        std::printf("NOTE: RELAX requested; volume relaxation is not part of this engine, skipped.\n");
+        return 0;
+    });
 }
 
 unsigned long getStepNum(struct MCState *m) { return m->sn; }

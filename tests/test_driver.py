@@ -150,3 +150,27 @@ def test_driver_replays_oracle(gpu, oracle, driver, tmp_path):
     rows = (tmp_path / "thermo.dat.mcs").read_text().splitlines()
     assert rows[0].startswith("Step") and len(rows) == 1 + 1 + 10   # header, step 0, 10 blocks
     assert len(rows[-1].split("\t")) == 13
+
+
+@pytest.mark.gpu
+def test_driver_team_sizes_give_identical_output(gpu, driver, tmp_path):
+    """The driver keeps the reference's OpenMP team structure (collective
+    incrementStep/Step/fad/printG, output in single regions): a team of 4
+    threads writes the same files as a team of 1."""
+    text = ("ENSEMBLE NPT\nN 200\nP 0.5\nT 1.0\nPOT LJshift 3.0\nMAXSTEP 0.5\nMAXDV 0.01\n"
+            "NUMSTEPS 402000\nCPI 100500\nTPI 20100\nSEED 3\nENGCHECK 40200\nBOUNDARY PERIODIC\n"
+            "VOLMOVE LOG\nLSTART 400\nFORM BATCHED_FP32\n")
+    outs = []
+    for threads in ("1", "4"):
+        d = tmp_path / f"t{threads}"
+        d.mkdir()
+        (d / "INPUT").write_text(text)
+        print(f"[gpu] driver OMP_NUM_THREADS={threads}", flush=True)
+        env = dict(os.environ, OMP_NUM_THREADS=threads)
+        r = subprocess.run([driver, str(d / "INPUT")], cwd=d, capture_output=True, text=True,
+                           timeout=120, env=env)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert f"Running with {threads} threads" in r.stdout
+        outs.append(((d / "thermo.dat.mcs").read_text(), (d / "config.dat.mcs").read_text()))
+    assert outs[0] == outs[1]
+    assert len(outs[0][0].splitlines()) == 1 + 1 + 20
