@@ -315,6 +315,10 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
     const double beta = ch.beta, gmin = ch.gmin, max_step = ch.max_step;
     const float betaf = ch.betaf;
     const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+    const uint64_t log_off = ch.log_off;
+    const uint32_t eci = ch.eci;
+    const uint32_t chain_id = ch.chain_id;
+    const double P = ch.P;
 
     const DevState st = a.states[ic];
     double L = st.L, U = st.U, mdrift = st.max_drift;
@@ -338,7 +342,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
             if (t < size) {
                 const uint32_t i = start + t;
                 uint32_t rr[4];
-                philox4x32_10(i, (uint32_t)sweep, ch.chain_id, 0u, key0, key1, rr);
+                philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
                 const double a1 = u53(rr[0], rr[1]);
                 const double u = u53(rr[2], rr[3]);
                 const double md = ((a1 - 0.5) * 2.0) * max_step;
@@ -373,7 +377,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
                         ++acc;
                     }
                 }
-                if (a.acclog) a.acclog[ch.log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
+                if (a.acclog) a.acclog[log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
             }
             __syncthreads();
         }
@@ -401,7 +405,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
         uint32_t accv = 0;
         if (npt) {
             ++n_vt;
-            if (vp.valid && !bad && vol_accept(vpart, vp.dL, vp.lns, ch.P, beta, vp.jac, vp.u)) {
+            if (vp.valid && !bad && vol_accept(vpart, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
                 for (uint32_t p = start; p < start + size; ++p) X[p] = X[p] * vp.sc;
                 L = vp.Lp;
                 U = U + vpart;
@@ -412,7 +416,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
         }
 
         // drift guard (ECheck, src/jmmMCState.cpp:1965-2095), always FP64
-        const int need = act && ch.eci && ((sweep + 1) % ch.eci == 0);
+        const int need = act && eci && ((sweep + 1) % eci == 0);
         if (__syncthreads_or(need)) {
             double v[1] = {need ? energy_part<KIND>(q, X, N, per, L, w, start, start + size, 1)
                                 : 0.0};
@@ -606,9 +610,15 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
     const DevChain &sh = a.chains[__builtin_amdgcn_readfirstlane(a.slots[bd.slot0].chain)];
     const Pot q = sh.pot;
     const double gmin = sh.gmin, max_step = sh.max_step;
+    // per-chain fields as values: read once, not reloaded after every store
+    // of the loops below (the compiler cannot rule out aliasing)
     const double beta = ch.beta;
     const float betaf = ch.betaf;
     const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+    const uint32_t chain_id = ch.chain_id;
+    const uint64_t log_off = ch.log_off;
+    const double P = ch.P;
+    const uint32_t eci = ch.eci;
 
     LdsChain c;
     c.R = smem + me.lds_off;
@@ -653,7 +663,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
                 const uint32_t i = start + t;
                 const uint32_t my = a0 + t;
                 uint32_t rr[4];
-                philox4x32_10(i, (uint32_t)sweep, ch.chain_id, 0u, key0, key1, rr);
+                philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
                 const double a1 = u53(rr[0], rr[1]);
                 const double u = u53(rr[2], rr[3]);
                 const double md = ((a1 - 0.5) * 2.0) * max_step;
@@ -673,7 +683,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
                     dUs += dE;
                     ++acc;
                 }
-                if (a.acclog) a.acclog[ch.log_off + (uint64_t)(soff + sw) * c.N + i] = accd ? 1 : 0;
+                if (a.acclog) a.acclog[log_off + (uint64_t)(soff + sw) * c.N + i] = accd ? 1 : 0;
             }
             __syncthreads();
         }
@@ -735,7 +745,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         uint32_t accv = 0;
         if (npt) {
             ++n_vt;
-            if (vp.valid && !bad && vol_accept(vpart, vp.dL, vp.lns, ch.P, beta, vp.jac, vp.u)) {
+            if (vp.valid && !bad && vol_accept(vpart, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
                 for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
                 L = vp.Lp;
                 U = U + vpart;
@@ -750,7 +760,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         }
 
         // drift guard (ECheck, src/jmmMCState.cpp:1965-2095), always FP64
-        const int need = act && ch.eci && ((sweep + 1) % ch.eci == 0);
+        const int need = act && eci && ((sweep + 1) % eci == 0);
         if (__syncthreads_or(need)) {
             double Up = 0.0;
             if (need) {
@@ -862,6 +872,10 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
     const bool npt = (ch.flags & F_NPT) != 0;
     const double beta = ch.beta, gmin = ch.gmin, max_step = ch.max_step;
     const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+    const uint64_t log_off = ch.log_off;
+    const double P = ch.P;
+    const uint32_t eci = ch.eci;
+    const uint32_t chain_id = ch.chain_id;
     double *X;
     if (GX) {
         X = a.x + ch.x_off;
@@ -887,7 +901,7 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
                 if (t >= size) continue;
                 const uint32_t i = b * qb + (b < rem ? b : rem) + t;
                 uint32_t rr[4];
-                philox4x32_10(i, (uint32_t)sweep, ch.chain_id, 0u, key0, key1, rr);
+                philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
                 const double a1 = u53(rr[0], rr[1]);
                 const double u = u53(rr[2], rr[3]);
                 const double md = ((a1 - 0.5) * 2.0) * max_step;
@@ -936,7 +950,7 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
                     __syncthreads();
                 }
                 if (a.acclog && lane == 0)
-                    a.acclog[ch.log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
+                    a.acclog[log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
             }
         }
         U = U + dUs;
@@ -952,7 +966,7 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
                                                          lane, (uint32_t)N, 64, bad);
                 part = wave_sum(part);
                 bad = wave_sum(bad);
-                if (!bad && vol_accept(part, vp.dL, vp.lns, ch.P, beta, vp.jac, vp.u)) {
+                if (!bad && vol_accept(part, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
                     __syncthreads();
                     for (uint32_t p = lane; p < (uint32_t)N; p += 64) X[p] = X[p] * vp.sc;
                     L = vp.Lp;
@@ -963,7 +977,7 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
                 __syncthreads();
             }
         }
-        if (ch.eci && ((sweep + 1) % ch.eci == 0)) {
+        if (eci && ((sweep + 1) % eci == 0)) {
             const double Uf = wave_sum(energy_part<KIND>(q, X, N, per, L, w, lane, (uint32_t)N, 64));
             const double dr = fabs(Uf - U);
             if (dr > mdrift) mdrift = dr;
