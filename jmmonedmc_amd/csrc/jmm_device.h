@@ -49,6 +49,12 @@ JMM_HD double u53(uint32_t a, uint32_t b)
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
+// Uniform in (0, 1) from one word: (w + 1/2) 2^-32, exact in FP64.
+JMM_HD double u32o(uint32_t w)
+{
+    return ((double)w + 0.5) * (1.0 / 4294967296.0);
+}
+
 // exp for the log-length move's scale factor, built only from IEEE
 // add/mul/floor so host and device agree bit for bit: Cody-Waite reduction
 // x = k ln2 + r with a two-constant ln2, degree-13 Taylor polynomial in

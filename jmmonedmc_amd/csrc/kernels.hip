@@ -338,13 +338,26 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
     for (uint32_t sw = 0; sw < a.n_sweeps; ++sw, ++sweep) {
         double dUs = 0.0;
         uint32_t acc = 0;
+        uint32_t w_md = 0, w_u = 0, w_next_md = 0, w_next_u = 0;
         for (uint32_t t = 0; t < bd.smax; ++t) {
             if (t < size) {
                 const uint32_t i = start + t;
-                uint32_t rr[4];
-                philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
-                const double a1 = u53(rr[0], rr[1]);
-                const double u = u53(rr[2], rr[3]);
+                // one Philox block per pair of block positions (t, t+1):
+                // computed at even t, its second half kept for odd t (t is
+                // the same on every lane, so the branch is uniform)
+                if ((t & 1u) == 0) {
+                    uint32_t rr[4];
+                    philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
+                    w_md = rr[0];
+                    w_u = rr[1];
+                    w_next_md = rr[2];
+                    w_next_u = rr[3];
+                } else {
+                    w_md = w_next_md;
+                    w_u = w_next_u;
+                }
+                const double a1 = u32o(w_md);
+                const double u = u32o(w_u);
                 const double md = ((a1 - 0.5) * 2.0) * max_step;
                 const double xi = X[i];
                 const double xn = xi + md;
@@ -657,15 +670,28 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
     for (uint32_t sw = 0; sw < a.n_sweeps; ++sw, ++sweep) {
         double dUs = 0.0;
         uint32_t acc = 0;
+        uint32_t w_md = 0, w_u = 0, w_next_md = 0, w_next_u = 0;
 #pragma unroll 1
         for (uint32_t t = 0; t < bd.smax; ++t) {
             if (t < size) {
                 const uint32_t i = start + t;
                 const uint32_t my = a0 + t;
-                uint32_t rr[4];
-                philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
-                const double a1 = u53(rr[0], rr[1]);
-                const double u = u53(rr[2], rr[3]);
+                // one Philox block per pair of block positions (t, t+1):
+                // computed at even t, its second half kept for odd t (t is
+                // the same on every lane, so the branch is uniform)
+                if ((t & 1u) == 0) {
+                    uint32_t rr[4];
+                    philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
+                    w_md = rr[0];
+                    w_u = rr[1];
+                    w_next_md = rr[2];
+                    w_next_u = rr[3];
+                } else {
+                    w_md = w_next_md;
+                    w_u = w_next_u;
+                }
+                const double a1 = u32o(w_md);
+                const double u = u32o(w_u);
                 const double md = ((a1 - 0.5) * 2.0) * max_step;
                 const double xi = c.R[my];
                 const double xn = xi + md;
@@ -901,9 +927,9 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
                 if (t >= size) continue;
                 const uint32_t i = b * qb + (b < rem ? b : rem) + t;
                 uint32_t rr[4];
-                philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
-                const double a1 = u53(rr[0], rr[1]);
-                const double u = u53(rr[2], rr[3]);
+                philox4x32_10(i - (t & 1u), (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
+                const double a1 = u32o((t & 1u) ? rr[2] : rr[0]);
+                const double u = u32o((t & 1u) ? rr[3] : rr[1]);
                 const double md = ((a1 - 0.5) * 2.0) * max_step;
                 const double xi = X[i];
                 const double xn = xi + md;

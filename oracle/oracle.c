@@ -51,6 +51,12 @@ double orc_u53(uint32_t a, uint32_t b)
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
+/* Uniform in (0, 1) from one word: (w + 1/2) 2^-32, exact in FP64. */
+double orc_u32(uint32_t w)
+{
+    return ((double)w + 0.5) * (1.0 / 4294967296.0);
+}
+
 /* Deterministic exp for state updates that must match bit for bit on CPU and
  * GPU (the log-length move's scale factor).  Method: Cody-Waite reduction
  * x = k ln2 + r, |r| <= ln2/2, two-constant ln2 split, degree-13 Taylor
@@ -433,9 +439,36 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
     return 0;
 }
 
-static inline void draw(const orc_chain *c, uint32_t slot, uint64_t sweep, double *a, double *b)
+/* Position t of particle i in its block (blocks of q+1 particles first,
+ * then blocks of q; q = N/B, N%B blocks of q+1). */
+static uint64_t block_pos(uint64_t N, uint64_t B, uint64_t i)
 {
-    uint32_t ctr[4] = { slot, (uint32_t)sweep, c->s.chain_id, 0u };
+    uint64_t q = N / B, r = N % B, big = r * (q + 1);
+    return i < big ? i % (q + 1) : (i - big) % q;
+}
+
+/* The uniforms of displacement trial i (block position t) in a sweep: one
+ * Philox4x32-10 block per pair of consecutive particles of a block,
+ * counter (first particle of the pair, sweep, chain_id, 0), key = seed; the
+ * pair's first particle (t even) takes words 0 (displacement) and 1
+ * (acceptance), the second words 2 and 3. */
+static inline void draw_trial(const orc_chain *c, uint64_t i, uint64_t t, uint64_t sweep,
+                              double *a, double *b)
+{
+    uint32_t ctr[4] = { (uint32_t)(i - (t & 1u)), (uint32_t)sweep, c->s.chain_id, 0u };
+    uint32_t key[2] = { (uint32_t)c->s.seed, (uint32_t)(c->s.seed >> 32) };
+    uint32_t r[4];
+    orc_philox4x32_10(ctr, key, r);
+    const int p = (int)(t & 1u) * 2;
+    *a = orc_u32(r[p]);
+    *b = orc_u32(r[p + 1]);
+}
+
+/* The uniforms of the box-length trial: counter (0xFFFFFFFF, sweep,
+ * chain_id, 0), 53-bit uniforms from words (0,1) and (2,3). */
+static inline void draw_volume(const orc_chain *c, uint64_t sweep, double *a, double *b)
+{
+    uint32_t ctr[4] = { 0xFFFFFFFFu, (uint32_t)sweep, c->s.chain_id, 0u };
     uint32_t key[2] = { (uint32_t)c->s.seed, (uint32_t)(c->s.seed >> 32) };
     uint32_t r[4];
     orc_philox4x32_10(ctr, key, r);
@@ -446,7 +479,7 @@ static inline void draw(const orc_chain *c, uint32_t slot, uint64_t sweep, doubl
 int orc_trial(orc_chain *c, uint64_t i, uint64_t sweep, double *dE_out)
 {
     double a, u;
-    draw(c, (uint32_t)i, sweep, &a, &u);
+    draw_trial(c, i, block_pos(c->s.N, c->B, i), sweep, &a, &u);
     double md = ((a - 0.5) * 2.0) * c->s.max_step; /* src/jmmMCState.cpp:1182 */
     return orc_trial_with(c, i, md, u, dE_out);
 }
@@ -515,7 +548,7 @@ int orc_volume_trial_with(orc_chain *c, double dv, double u, double *dU_out)
 int orc_volume_trial(orc_chain *c, uint64_t sweep, double *dU_out)
 {
     double a, u;
-    draw(c, 0xFFFFFFFFu, sweep, &a, &u);
+    draw_volume(c, sweep, &a, &u);
     double dv = ((a - 0.5) * 2.0) * c->s.max_dv; /* src/jmmMCState.cpp:2172 */
     return orc_volume_trial_with(c, dv, u, dU_out);
 }
@@ -687,7 +720,7 @@ static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t 
                 if (t >= size) continue;
                 uint64_t i = start + t;
                 double a, u;
-                draw(c, (uint32_t)i, sweep, &a, &u);
+                draw_trial(c, i, t, sweep, &a, &u);
                 double md = ((a - 0.5) * 2.0) * s->max_step;
                 int acc = allpairs ? trial_allpairs(c, i, md, u) : orc_trial_with(c, i, md, u, NULL);
                 if (log) log[it * N + i] = (uint8_t)acc;

@@ -57,6 +57,8 @@ def lib():
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_u53.argtypes = [C.c_uint32, C.c_uint32]
         L.orc_u53.restype = C.c_double
+        L.orc_u32.argtypes = [C.c_uint32]
+        L.orc_u32.restype = C.c_double
         L.orc_det_exp.argtypes = [C.c_double]
         L.orc_det_exp.restype = C.c_double
         L.orc_phi.argtypes = [C.POINTER(Spec), C.c_double, C.c_double, dp]
@@ -236,6 +238,10 @@ def run_batch(specs, n_sweeps, n_threads, allpairs=False):
 
 def u53(a, b):
     return lib().orc_u53(a, b)
+
+
+def u32(w):
+    return lib().orc_u32(w)
 
 
 def det_exp(x):

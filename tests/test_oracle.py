@@ -39,6 +39,42 @@ def test_u53_range(oracle):
     assert all(x < y for x, y in zip(v, v[1:]))
 
 
+def test_u32_range(oracle):
+    assert oracle.u32(0) == 2.0 ** -33
+    assert oracle.u32(0xFFFFFFFF) == 1.0 - 2.0 ** -33
+    assert oracle.u32(0x80000000) == 0.5 + 2.0 ** -33
+    assert oracle.u32(12345) == (12345 + 0.5) / 2.0 ** 32
+
+
+def test_trial_stream_is_pairwise_philox(oracle):
+    """orc_trial's uniforms: for a block pair (t even, t+1) of particles
+    (i, i+1), one Philox block at counter (i, sweep, chain_id, 0); words 0/1
+    drive particle i (displacement, acceptance), words 2/3 particle i+1.
+    Replayed by hand through orc_trial_with on a copy of the chain."""
+    from jmmonedmc_amd.engine import Chain
+    from helpers import lj, oracle_spec
+    ch = Chain(N=37, L0=74.0, seed=0x1234567890AB, chain_id=9, bcap=4, **lj())
+    spec = oracle_spec(oracle, ch)
+    a, b = oracle.Chain(spec), oracle.Chain(spec)
+    B = a.counter("B")
+    q, rem = ch.N // B, ch.N % B
+    key = (0x567890AB, 0x1234)
+    sweep = 0
+    for t in range(q + 1):
+        for blk in range(B):
+            size = q + (blk < rem)
+            if t >= size:
+                continue
+            i = blk * q + min(blk, rem) + t
+            r = oracle.philox((i - (t & 1), sweep, 9, 0), key)
+            p = 2 * (t & 1)
+            md = ((oracle.u32(r[p]) - 0.5) * 2.0) * ch.max_step
+            acc_a, dEa = a.trial(i, sweep)
+            acc_b, dEb = b.trial_with(i, md, oracle.u32(r[p + 1]))
+            assert (acc_a, dEa) == (acc_b, dEb), (t, blk, i)
+    assert np.array_equal(a.state()[0], b.state()[0])
+
+
 def test_det_exp_against_libm(oracle):
     xs = np.concatenate([np.linspace(-0.02, 0.02, 2001), np.linspace(-700, 700, 2001),
                          [0.0, 1e-300, -1e-300, 0.5 * math.log(2), -0.5 * math.log(2)]])
