@@ -532,48 +532,47 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
                                           uint32_t &n_core)
 {
     const double *X = c.R;
-    if (!c.per && fabs(xn) > L / 2.0) {
-        ++n_wall;
-        return false;
-    }
+    // rejections by the walls and the hard core are flags, not early exits:
+    // the wavefront walks the window anyway for its other lanes
+    const bool wall = !c.per && fabs(xn) > L / 2.0;
     const bool hasL = c.N >= 2 && (c.per || i > 0);
     const bool hasR = c.N >= 2 && (c.per || i + 1 < c.N);
     const double xl = X[hasL ? my - 1 - (t == 0 ? gapL : 0) : my];
     const double xr = X[hasR ? my + 1 + (t + 1 >= size ? gapR : 0) : my];
-    if ((hasL && (xn - xl) < gmin) || (hasR && (xr - xn) < gmin)) {
-        ++n_core;
-        return false;
-    }
+    const bool core = !wall && ((hasL && (xn - xl) < gmin) || (hasR && (xr - xn) < gmin));
+    n_wall += wall ? 1u : 0u;
+    n_core += core ? 1u : 0u;
+    const bool go = !wall && !core;
     const uint32_t kL = c.per ? c.w : min(c.w, i);
     const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
-    bool liveL = kL >= 1, liveR = kR >= 1;
+    bool liveL = go && kL >= 1, liveR = go && kR >= 1;
     if (sizeof(Real) == 8) {
         double dEL = 0.0, dER = 0.0;
 #pragma unroll 1
         for (uint32_t k = 1; __any(liveL || liveR); ++k) {
-            const double xjL = k == 1 ? xl : X[liveL ? my - k - (k > t ? gapL : 0) : my];
-            const double xjR = k == 1 ? xr : X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
+            const double xjL = X[liveL ? my - k - (k > t ? gapL : 0) : my];
+            const double xjR = X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
             const double oL = xi - xjL, nL = xn - xjL;
             const double oR = xjR - xi, nR = xjR - xn;
             const bool inL = liveL && !(oL > q.rc && nL > q.rc);
             const bool inR = liveR && !(oR > q.rc && nR > q.rc);
             const double tL = phi64<KIND>(q, nL) - phi64<KIND>(q, oL);
             const double tR = phi64<KIND>(q, nR) - phi64<KIND>(q, oR);
-            if (inL) dEL += tL;
-            if (inR) dER += tR;
+            dEL += inL ? tL : 0.0;
+            dER += inR ? tR : 0.0;
             terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
             liveL = inL && k < kL;
             liveR = inR && k < kR;
         }
         dE = dEL + dER;
-        if (dE <= 0.0) return true;
-        return exp(-dE * beta) > u;
+        const double bf = exp(-dE * beta);
+        return go && (dE <= 0.0 || bf > u);
     }
     float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
 #pragma unroll 1
     for (uint32_t k = 1; __any(liveL || liveR); ++k) {
-        const double xjL = k == 1 ? xl : X[liveL ? my - k - (k > t ? gapL : 0) : my];
-        const double xjR = k == 1 ? xr : X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
+        const double xjL = X[liveL ? my - k - (k > t ? gapL : 0) : my];
+        const double xjR = X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
         // separations are FP64 differences; the cutoff decision is made on
         // them (the FP64 energies of set-up and recompute count the same
         // pairs), the pair energies on their FP32 roundings
@@ -599,8 +598,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         liveR = inR && k < kR;
     }
     dE = ((double)sL - (double)cL) + ((double)sR - (double)cR);
-    if (dE <= 0.0) return true;
-    return (double)fexp(-betaf * (float)dE) > u;
+    const float bf = fexp(-betaf * (float)dE);
+    return go && (dE <= 0.0 || (double)bf > u);
 }
 
 template <typename Real, int KIND, int T>
@@ -700,15 +699,14 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
                 const bool accd = trial_lds<Real, KIND>(c, q, gmin, beta, betaf, i, t, size, my,
                                                         gapL, gapR, L, xi, xn, md, u, dE, n_terms,
                                                         n_wall, n_core);
-                if (accd) {
-                    c.R[my] = xn;
-                    if (c.per) {
-                        if (i < c.wg) c.R[c.wg + c.B * c.Sp + i] = xn + L;
-                        if (i + c.wg >= c.N) c.R[i + c.wg - c.N] = xn - L;
-                    }
-                    dUs += dE;
-                    ++acc;
+                const double xs = accd ? xn : xi;
+                c.R[my] = xs;
+                if (c.per) {
+                    if (i < c.wg) c.R[c.wg + c.B * c.Sp + i] = xs + L;
+                    if (i + c.wg >= c.N) c.R[i + c.wg - c.N] = xs - L;
                 }
+                dUs += accd ? dE : 0.0;
+                acc += accd ? 1u : 0u;
                 if (a.acclog) a.acclog[log_off + (uint64_t)(soff + sw) * c.N + i] = accd ? 1 : 0;
             }
             __syncthreads();

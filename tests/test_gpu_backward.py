@@ -189,7 +189,10 @@ def test_engine_observables_and_reweighting(gpu, oracle):
     base, _ = _device_obs(chains[0], X, L, 64)
     beta = 1.0
     fg = fluctuation_gradient(base[:, 1], base[:, 4:6], base[:, 2:4], beta)
-    h = 1e-4
+    # the identity is exact as h -> 0; central differences leave h^2 terms of
+    # order (beta h dU/dtheta)^2, so h is kept small (FP64 U: no cancellation
+    # problem at this size)
+    h = 1e-5
     for k in range(2):
         vals = []
         for sgn in (1, -1):
@@ -199,4 +202,4 @@ def test_engine_observables_and_reweighting(gpu, oracle):
             wgt = np.exp(-beta * (o[:, 0] - base[:, 0]))
             vals.append((o[:, 1] * wgt).sum() / wgt.sum())
         fd = (vals[0] - vals[1]) / (2 * h)
-        assert fg[k] == pytest.approx(fd, rel=TOL_FD, abs=1e-6), (k, fg[k], fd)
+        assert fg[k] == pytest.approx(fd, rel=1e-5, abs=1e-6), (k, fg[k], fd)
