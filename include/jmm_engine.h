@@ -116,6 +116,16 @@ typedef struct jmm_counters {
     double L;               /* current box length */
 } jmm_counters;
 
+/* Density and pair-separation histograms (reference RHONB/RBW, GNS/GSW,
+ * GNB/GBW; src/jmmMCState.h:60-61, :128-132). */
+typedef struct jmm_hist_spec {
+    uint32_t rhonb;  /* rho(x) bins, width rbw, centred on x = 0 */
+    uint32_t gns;    /* g(x) segments of width gsw, centred on x = 0 (0 = no g(x)) */
+    uint32_t gnb;    /* g(x) separation bins of width gbw */
+    uint32_t reserved;
+    double rbw, gsw, gbw;
+} jmm_hist_spec;
+
 typedef struct jmm_engine jmm_engine;
 
 /* Library identity; no reference counterpart. */
@@ -190,6 +200,28 @@ int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out);
  * sum_c N_c * max_sweeps bytes; chain c starts at offset
  * max_sweeps * sum_{c'<c} N_c'. */
 int jmm_read_accept_log(jmm_engine *e, uint8_t *out, uint32_t max_sweeps, uint32_t *n_out);
+
+/* Replaces the histogram bookkeeping of fgrho/ugrho/qagrho
+ * (src/jmmMCState.cpp:1069-1149, :2297-2384): every `every` sweeps (counted
+ * from creation) each chain's current configuration is binned: rho bin
+ * floor(x/rbw + rhonb/2) per particle, and for every pair closer than
+ * gnb*gbw the separation bin floor(d/gbw) once in the segment
+ * floor(x/gsw + gns/2) of each member (periodic chains: positions folded into
+ * [-L/2, L/2), each pair at its separation below L/2).  The reference samples
+ * after every trial; the engine samples configurations between sweeps.
+ * every = 0 switches sampling off.  Clears all counts.  Synchronous. */
+int jmm_hist_configure(jmm_engine *e, const jmm_hist_spec *spec, uint32_t every);
+
+/* One extra sample of the current configurations now (the reference's
+ * step-0 fgrho/ugrho, src/Main.cpp:66-68 via fad).  Asynchronous. */
+int jmm_hist_sample(jmm_engine *e);
+
+/* Replaces printRho/printG's accumulators (src/jmmMCState.cpp:1021-1060):
+ * rho[c*rhonb + b] and g[(c*gns + s)*gnb + b] receive the counts summed over
+ * the *n_samples samples since the last reset (the reference prints
+ * rhoA/ns/rbw and gA/ns/gsw/gbw).  Either array may be NULL.  reset = 1
+ * clears the counts.  Synchronous. */
+int jmm_hist_read(jmm_engine *e, uint64_t *rho, uint64_t *g, uint64_t *n_samples, int reset);
 
 /* Kernel timing of the sweep kernel, from HIP events recorded on the
  * engine's stream around every sweep-block launch since the last reset:

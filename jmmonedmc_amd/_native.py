@@ -23,6 +23,7 @@ EXPORTED = [
     "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples",
     "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
+    "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read",
 ]
 
 
@@ -47,6 +48,12 @@ class EngineOpts(C.Structure):
 
 class Sample(C.Structure):
     _fields_ = [("U", C.c_double), ("L", C.c_double), ("acc_d", C.c_uint32), ("acc_v", C.c_uint32)]
+
+
+class HistSpec(C.Structure):
+    _fields_ = [("rhonb", C.c_uint32), ("gns", C.c_uint32), ("gnb", C.c_uint32),
+                ("reserved", C.c_uint32), ("rbw", C.c_double), ("gsw", C.c_double),
+                ("gbw", C.c_double)]
 
 
 class Counters(C.Structure):
@@ -94,6 +101,9 @@ def lib(path: str = LIB):
     L.jmm_kernel_time.argtypes = [vp, u64p, dp, u64p, C.c_int]
     L.jmm_observables.argtypes = [vp, dp, dp]
     L.jmm_pair_observables.argtypes = [C.POINTER(ChainSpec), C.c_int, vp, vp, vp, vp, C.c_int, vp]
+    L.jmm_hist_configure.argtypes = [vp, C.POINTER(HistSpec), C.c_uint32]
+    L.jmm_hist_read.argtypes = [vp, u64p, u64p, u64p, C.c_int]
+    L.jmm_hist_sample.argtypes = [vp]
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name == "jmm_last_error" else C.c_int
     _lib = L

@@ -270,6 +270,12 @@ struct jmm_engine {
     hipGraph_t graph = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     uint64_t ev_sweeps = 0;
+    uint64_t sweeps_run = 0;            // sweeps launched since creation
+    // rho(x)/g(x) sampling (jmm_hist_configure)
+    uint32_t hist_every = 0;
+    jmm_hist_spec hs{};
+    unsigned long long *d_rho = nullptr, *d_g = nullptr;
+    uint64_t hist_samples = 0;
 };
 
 namespace {
@@ -436,6 +442,8 @@ int jmm_destroy(jmm_engine *e)
     (void)hipFree(e->d_grad);
     (void)hipFree(e->d_Ltmp);
     (void)hipFree(e->d_all);
+    (void)hipFree(e->d_rho);
+    (void)hipFree(e->d_g);
     for (auto &G : e->grp) {
         (void)hipFree(G.d_bundles);
         (void)hipFree(G.d_list);
@@ -618,6 +626,93 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     return 0;
 }
 
+static int launch_hist_sample(jmm_engine *e)
+{
+    HistArgs h{};
+    h.chains = e->d_chains;
+    h.states = e->d_states;
+    h.x = e->d_x;
+    h.rho = e->d_rho;
+    h.g = e->d_g;
+    h.rbw = e->hs.rbw;
+    h.gsw = e->hs.gsw;
+    h.gbw = e->hs.gbw;
+    h.n_chains = (uint32_t)e->n;
+    h.rhonb = e->hs.rhonb;
+    h.gns = e->hs.gns;
+    h.gnb = e->hs.gnb;
+    uint64_t nmax = 1;
+    for (const auto &c : e->hch) nmax = std::max<uint64_t>(nmax, c.N);
+    h.parts = (uint32_t)std::min<uint64_t>(64, (nmax + 255) / 256);
+    HIPCHK(launch_hist(h, e->stream));
+    e->hist_samples++;
+    return 0;
+}
+
+int jmm_hist_configure(jmm_engine *e, const jmm_hist_spec *spec, uint32_t every)
+{
+    if (!e || (every && !spec)) return fail(JMM_EINVAL, "jmm_hist_configure: null argument");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    (void)hipFree(e->d_rho);
+    (void)hipFree(e->d_g);
+    e->d_rho = nullptr;
+    e->d_g = nullptr;
+    e->hist_every = 0;
+    e->hist_samples = 0;
+    if (!every) return 0;
+    const jmm_hist_spec &h = *spec;
+    if (h.rhonb == 0 || !(h.rbw > 0.0) || (h.gns && (!(h.gsw > 0.0) || h.gnb == 0 || !(h.gbw > 0.0))))
+        return fail(JMM_EINVAL, "jmm_hist_configure: bin counts and widths must be positive");
+    const uint64_t nr = (uint64_t)e->n * h.rhonb, ng = (uint64_t)e->n * h.gns * h.gnb;
+    if (int rc = dalloc(&e->d_rho, nr)) return rc;
+    if (ng)
+        if (int rc = dalloc(&e->d_g, ng)) return rc;
+    HIPCHK(hipMemsetAsync(e->d_rho, 0, sizeof(unsigned long long) * nr, e->stream));
+    if (ng) HIPCHK(hipMemsetAsync(e->d_g, 0, sizeof(unsigned long long) * ng, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->hs = h;
+    e->hist_every = every;
+    return 0;
+}
+
+int jmm_hist_sample(jmm_engine *e)
+{
+    if (!e) return fail(JMM_EINVAL, "jmm_hist_sample: null engine");
+    if (!e->hist_every) return fail(JMM_ESTATE, "jmm_hist_sample: histograms are not configured");
+    HIPCHK(hipSetDevice(e->device));
+    return launch_hist_sample(e);
+}
+
+int jmm_hist_read(jmm_engine *e, uint64_t *rho, uint64_t *g, uint64_t *n_samples, int reset)
+{
+    if (!e || !n_samples) return fail(JMM_EINVAL, "jmm_hist_read: null argument");
+    if (!e->hist_every) return fail(JMM_ESTATE, "jmm_hist_read: histograms are not configured");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const uint64_t rn = e->hs.rhonb, gn = (uint64_t)e->hs.gns * e->hs.gnb;
+    if (rho) {
+        std::vector<unsigned long long> h((size_t)e->n * rn);
+        HIPCHK(hipMemcpy(h.data(), e->d_rho, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < e->n; ++k)
+            for (uint64_t b = 0; b < rn; ++b) rho[(uint64_t)e->ext_of_int[k] * rn + b] = h[(uint64_t)k * rn + b];
+    }
+    if (g && gn) {
+        std::vector<unsigned long long> h((size_t)e->n * gn);
+        HIPCHK(hipMemcpy(h.data(), e->d_g, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < e->n; ++k)
+            for (uint64_t b = 0; b < gn; ++b) g[(uint64_t)e->ext_of_int[k] * gn + b] = h[(uint64_t)k * gn + b];
+    }
+    *n_samples = e->hist_samples;
+    if (reset) {
+        HIPCHK(hipMemsetAsync(e->d_rho, 0, sizeof(unsigned long long) * e->n * rn, e->stream));
+        if (gn) HIPCHK(hipMemsetAsync(e->d_g, 0, sizeof(unsigned long long) * e->n * gn, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->hist_samples = 0;
+    }
+    return 0;
+}
+
 int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps)
 {
     if (!e) return fail(JMM_EINVAL, "jmm_run_sweeps: null engine");
@@ -625,7 +720,10 @@ int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps)
         return fail(JMM_EFULL, "sample buffer holds %u sweeps, %u pending: read samples first", e->cap, e->write_pos);
     HIPCHK(hipSetDevice(e->device));
     while (n_sweeps > 0) {
-        const uint32_t m = std::min(n_sweeps, e->block_sweeps);
+        uint32_t m = std::min(n_sweeps, e->block_sweeps);
+        // with histograms on, blocks end on every sampling sweep
+        if (e->hist_every)
+            m = std::min<uint32_t>(m, e->hist_every - (uint32_t)(e->sweeps_run % e->hist_every));
         hipEvent_t a, b;
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));
@@ -639,7 +737,11 @@ int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps)
         HIPCHK(hipEventRecord(b, e->stream));
         e->ev_sweeps += m;
         e->write_pos += m;
+        e->sweeps_run += m;
         n_sweeps -= m;
+        if (e->hist_every && e->sweeps_run % e->hist_every == 0) {
+            if (int rc = launch_hist_sample(e)) return rc;
+        }
     }
     return 0;
 }

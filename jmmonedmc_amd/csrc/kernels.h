@@ -90,6 +90,17 @@ struct ObsArgs {
     uint32_t standalone;
 };
 
+// rho(x) and g(x) histograms of every chain's current configuration.
+struct HistArgs {
+    const DevChain *chains;
+    const DevState *states;
+    const double *x;
+    unsigned long long *rho;  // [chain][rhonb]
+    unsigned long long *g;    // [chain][gns][gnb]
+    double rbw, gsw, gbw;
+    uint32_t n_chains, rhonb, gns, gnb, parts;
+};
+
 }  // namespace jmm
 
 namespace jmm {
@@ -101,4 +112,5 @@ const void *plain_kernel(int gx);
 hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s);
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s);
+hipError_t launch_hist(const HistArgs &a, hipStream_t s);
 }  // namespace jmm

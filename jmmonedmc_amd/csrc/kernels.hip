@@ -1145,6 +1145,65 @@ __global__ void k_lengths(const DevState *st, double *L, uint32_t n)
     if (i < n) L[i] = st[i].L;
 }
 
+// ------------------------------------------------------------ histograms
+
+// Position folded into [-L/2, L/2) on a periodic line; unchanged with walls.
+__device__ __forceinline__ double fold(double x, double L, bool per)
+{
+    return per ? x - L * floor((x + L / 2.0) / L) : x;
+}
+
+__device__ __forceinline__ int64_t bin_of(double x, double width, uint32_t nb)
+{
+    return (int64_t)floor(x / width + nb / 2.0);
+}
+
+// One sample of the reference's density and pair-separation histograms
+// (fgrho, src/jmmMCState.cpp:1069-1128): rho bin floor(x/rbw + rhonb/2) of
+// every particle; for every pair closer than gnb*gbw, separation bin
+// floor(d/gbw) counted once in the segment floor(x/gsw + gns/2) of each
+// member.  Periodic chains: positions folded into [-L/2, L/2), each pair
+// once at its shorter separation (below L/2).  Grid: chains x parts, the
+// parts of a chain stride over its particles.
+__global__ void __launch_bounds__(256) k_hist(HistArgs a)
+{
+    const uint32_t c = blockIdx.x / a.parts, part = blockIdx.x % a.parts;
+    const DevChain &ch = a.chains[c];
+    const uint32_t N = ch.N;
+    const bool per = (ch.flags & F_PERIODIC) != 0;
+    const double L = a.states[c].L;
+    const double *X = a.x + ch.x_off;
+    const double gmax = a.gnb * a.gbw;
+    unsigned long long *rho = a.rho + (uint64_t)c * a.rhonb;
+    unsigned long long *g = a.g + (uint64_t)c * a.gns * a.gnb;
+    for (uint32_t i = part * 256 + threadIdx.x; i < N; i += 256 * a.parts) {
+        const double xi = X[i];
+        const double fi = fold(xi, L, per);
+        const int64_t rb = bin_of(fi, a.rbw, a.rhonb);
+        if (rb >= 0 && rb < (int64_t)a.rhonb) atomicAdd(&rho[rb], 1ull);
+        if (!a.gns || !a.gnb) continue;
+        const int64_t si = bin_of(fi, a.gsw, a.gns);
+        for (uint32_t k = 1; k < N; ++k) {
+            const uint32_t j0 = i + k;
+            double d, xj;
+            if (j0 < N) {
+                xj = X[j0];
+                d = xj - xi;
+            } else {
+                if (!per) break;
+                xj = X[j0 - N];
+                d = (xj + L) - xi;
+            }
+            if (!(d < gmax) || (per && !(d < L / 2.0))) break;
+            const int64_t gb = (int64_t)floor(d / a.gbw);
+            if (gb < 0 || gb >= (int64_t)a.gnb) continue;
+            const int64_t sj = bin_of(fold(xj, L, per), a.gsw, a.gns);
+            if (si >= 0 && si < (int64_t)a.gns) atomicAdd(&g[(uint64_t)si * a.gnb + gb], 1ull);
+            if (sj >= 0 && sj < (int64_t)a.gns) atomicAdd(&g[(uint64_t)sj * a.gnb + gb], 1ull);
+        }
+    }
+}
+
 // ------------------------------------------------------------ launchers
 
 hipError_t launch_begin(Ctrl *c, uint32_t n, hipStream_t s)
@@ -1191,6 +1250,12 @@ hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStre
 {
     if (fp32) hipLaunchKernelGGL(k_observables<float>, dim3(grid), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_observables<double>, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist(const HistArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hist, dim3(a.n_chains * a.parts), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as nat
-from ._native import ChainSpec, Counters, EngineOpts, Sample, check, lib
+from ._native import ChainSpec, Counters, EngineOpts, HistSpec, Sample, check, lib
 
 
 @dataclass
@@ -184,6 +184,29 @@ class Engine:
         if not want_grad:
             return obs
         return obs, [g[o: o + n] for o, n in zip(self.offsets, self.Ns)]
+
+    def hist_configure(self, every: int, rhonb: int = 0, rbw: float = 0.0, gns: int = 0,
+                       gsw: float = 0.0, gnb: int = 0, gbw: float = 0.0):
+        """Sample rho(x) and g(x) every `every` sweeps (0 = off)."""
+        h = HistSpec()
+        h.rhonb, h.rbw, h.gns, h.gsw, h.gnb, h.gbw = rhonb, rbw, gns, gsw, gnb, gbw
+        self._hist = (rhonb, gns, gnb)
+        check(lib().jmm_hist_configure(self.h, C.byref(h), every), "jmm_hist_configure")
+
+    def hist_sample(self):
+        check(lib().jmm_hist_sample(self.h), "jmm_hist_sample")
+
+    def hist_read(self, reset: bool = False):
+        """(rho[chain, rhonb], g[chain, gns, gnb], n_samples) counts."""
+        rhonb, gns, gnb = self._hist
+        rho = np.zeros(self.n * rhonb, dtype=np.uint64)
+        g = np.zeros(max(self.n * gns * gnb, 1), dtype=np.uint64)
+        ns = C.c_uint64()
+        u64p = C.POINTER(C.c_uint64)
+        check(lib().jmm_hist_read(self.h, rho.ctypes.data_as(u64p), g.ctypes.data_as(u64p),
+                                  C.byref(ns), int(reset)), "jmm_hist_read")
+        return (rho.reshape(self.n, rhonb), g[: self.n * gns * gnb].reshape(self.n, gns, gnb),
+                ns.value)
 
     def kernel_time(self, reset: bool = False):
         n, ms, sw = C.c_uint64(), C.c_double(), C.c_uint64()

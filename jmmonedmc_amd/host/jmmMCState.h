@@ -23,8 +23,8 @@ void printStep(struct MCState *mcs);            // :845
 int printMCP(struct MCState *mcs);              // :216
 int printThermo(struct MCState *mcs);           // :1896
 int printCoords(struct MCState *mcs);           // :1007
-int printRho(struct MCState *mcs);              // :1021 (histograms: not produced)
-int printG(struct MCState *mcs);                // :1042 (histograms: not produced)
+int printRho(struct MCState *mcs);              // :1021 rho.dat.mcs
+int printG(struct MCState *mcs);                // :1042 g<s>.dat.mcs
 int fad(struct MCState *mcs, unsigned long *nm, double *d);  // :853 (initial energy)
 unsigned long incrementStep(struct MCState *mcs);            // :1734
 int Step(struct MCState *mcs);                  // :1758

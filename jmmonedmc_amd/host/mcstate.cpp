@@ -46,6 +46,13 @@ struct MCState {
     double E = 0, l = 0, Vir = 0, HV = 0;
     FILE *cf = nullptr, *tf = nullptr;
     std::vector<jmm_sample> buf;
+    // rho(x)/g(x): engine counts pulled into per-file accumulators, so the
+    // two print intervals reset independently as in the reference
+    bool hist = false;
+    FILE *rhof = nullptr;
+    std::vector<FILE *> gf;
+    std::vector<uint64_t> rhoAcc, gAcc, tmpR, tmpG;
+    uint64_t nsRho = 0, nsG = 0;
 };
 
 static void die_on(int rc, const char *what)
@@ -193,6 +200,35 @@ struct MCState *setupMCS(struct MCInput inp)
         return nullptr;
     }
     printMCP(m);
+    if ((inp.rhopi && inp.rhonb && inp.rbw > 0.0) || (inp.gpi && inp.gns > 0 && inp.gnb && inp.gbw > 0.0)) {
+        jmm_hist_spec hs{};
+        hs.rhonb = (uint32_t)std::max(1ul, inp.rhonb);
+        hs.rbw = inp.rbw > 0.0 ? inp.rbw : 1.0;
+        const bool g = inp.gns > 0 && inp.gnb && inp.gbw > 0.0 && inp.gsw > 0.0;
+        hs.gns = g ? (uint32_t)inp.gns : 0;
+        hs.gsw = g ? inp.gsw : 0.0;
+        hs.gnb = g ? (uint32_t)inp.gnb : 0;
+        hs.gbw = g ? inp.gbw : 0.0;
+        rc = jmm_hist_configure(m->eng, &hs, (uint32_t)std::max(1ul, inp.histEvery));
+        if (rc) {
+            die_on(rc, "jmm_hist_configure");
+        } else {
+            m->hist = true;
+            const size_t nc = m->specs.size();
+            m->rhoAcc.assign(hs.rhonb, 0);
+            m->gAcc.assign((size_t)hs.gns * hs.gnb, 0);
+            m->tmpR.assign(nc * hs.rhonb, 0);
+            m->tmpG.assign(std::max<size_t>(1, nc * hs.gns * hs.gnb), 0);
+            const char *mode = inp.isRestart ? "a" : "w";
+            m->rhof = std::fopen("rho.dat.mcs", mode);
+            for (uint32_t k = 0; k < hs.gns; ++k) {
+                char name[64];
+                std::snprintf(name, sizeof name, "g%u.dat.mcs", k);
+                m->gf.push_back(std::fopen(name, mode));
+            }
+            jmm_hist_sample(m->eng);  // the configuration at the start
+        }
+    }
     jmm_counters c{};
     jmm_get_counters(m->eng, 0, &c);
     m->E = c.U;
@@ -262,6 +298,9 @@ void freeMCS(struct MCState *m)
     if (!m) return;
     if (m->cf) std::fclose(m->cf);
     if (m->tf) std::fclose(m->tf);
+    if (m->rhof) std::fclose(m->rhof);
+    for (FILE *f : m->gf)
+        if (f) std::fclose(f);
     jmm_destroy(m->eng);
     delete m;
 }
@@ -383,8 +422,63 @@ int printCoords(struct MCState *m)
     return 0;
 }
 
-int printRho(struct MCState *) { return 0; }
-int printG(struct MCState *) { return collective<int>([] { return 0; }); }
+// Runs the queued sweeps and moves the engine's counts (chain 0, the
+// reference's single chain) into both accumulators.
+static int pull_hist(MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    uint64_t ns = 0;
+    int rc = jmm_hist_read(m->eng, m->tmpR.data(), m->tmpG.data(), &ns, 1);
+    if (rc) return rc;
+    for (size_t b = 0; b < m->rhoAcc.size(); ++b) m->rhoAcc[b] += m->tmpR[b];
+    for (size_t b = 0; b < m->gAcc.size(); ++b) m->gAcc[b] += m->tmpG[b];
+    m->nsRho += ns;
+    m->nsG += ns;
+    return 0;
+}
+
+// src/jmmMCState.cpp:1021-1037: one row "step rho_0 ... rho_{rhonb-1}" with
+// rho_b = counts / samples / rbw.
+int printRho(struct MCState *m)
+{
+    if (!m->hist || !m->rhof) return 0;
+    if (int rc = pull_hist(m)) return rc;
+    std::fprintf(m->rhof, "%lu", m->sn);
+    const double ns = m->nsRho ? (double)m->nsRho : 1.0;
+    for (size_t b = 0; b < m->rhoAcc.size(); ++b) {
+        std::fprintf(m->rhof, " %.8G", (double)m->rhoAcc[b] / ns / m->inp.rbw);
+        m->rhoAcc[b] = 0;
+    }
+    std::fprintf(m->rhof, "\n");
+    std::fflush(m->rhof);
+    m->nsRho = 0;
+    return 0;
+}
+
+// src/jmmMCState.cpp:1042-1062: per segment s a row "step g_0 ... g_{gnb-1}"
+// in g<s>.dat.mcs with g_b = counts / samples / gsw / gbw.
+int printG(struct MCState *m)
+{
+    return collective<int>([m] {
+        if (!m->hist || m->gf.empty()) return 0;
+        if (int rc = pull_hist(m)) return rc;
+        const size_t gnb = m->inp.gnb;
+        const double ns = m->nsG ? (double)m->nsG : 1.0;
+        for (size_t k = 0; k < m->gf.size(); ++k) {
+            FILE *f = m->gf[k];
+            if (!f) continue;
+            std::fprintf(f, "%lu", m->sn);
+            for (size_t b = 0; b < gnb; ++b) {
+                std::fprintf(f, " %.8G", (double)m->gAcc[k * gnb + b] / ns / m->inp.gsw / m->inp.gbw);
+                m->gAcc[k * gnb + b] = 0;
+            }
+            std::fprintf(f, "\n");
+            std::fflush(f);
+        }
+        m->nsG = 0;
+        return 0;
+    });
+}
 
 int ECheck(struct MCState *m)
 {

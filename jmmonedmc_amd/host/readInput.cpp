@@ -139,6 +139,8 @@ bool readInputFile(const char *fstr, MCInput &inp)
             inp.chains = unum(t, 1);
         } else if (k == "SWEEPBLOCK") {
             inp.sweepBlock = unum(t, 1);
+        } else if (k == "HISTEVERY") {
+            inp.histEvery = std::max(1ul, unum(t, 1));
         } else if (k == "BCAP") {
             inp.bcap = unum(t, 1);
         } else if (k == "LSTART") {
@@ -198,6 +200,7 @@ int printInputFields(const MCInput &i)
                 i.nbn, i.numComputes);
     std::printf("boundary=%s\nvolMove=%s\nform=%s\nchains=%lu\nsweepBlock=%lu\nbcap=%lu\nlStart=%.17g\n",
                 i.boundary.c_str(), i.volMove.c_str(), i.form.c_str(), i.chains, i.sweepBlock, i.bcap, i.lStart);
+    std::printf("histEvery=%lu\n", i.histEvery);
     std::printf("potParams=%.17g,%.17g,%.17g,%.17g\npotParamsSet=%d\n", i.potParams[0], i.potParams[1],
                 i.potParams[2], i.potParams[3], i.potParamsSet);
     return 0;

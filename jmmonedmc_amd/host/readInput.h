@@ -43,6 +43,7 @@ struct MCInput {
     int potParamsSet = 0;              // POTPARAMS p0 [p1 [p2 [p3]]]
     unsigned long chains = 1;          // CHAINS: independent replicas, seeds SEED+k
     unsigned long sweepBlock = 100;    // SWEEPBLOCK: sweeps per kernel launch
+    unsigned long histEvery = 1;       // HISTEVERY: sweeps between rho(x)/g(x) samples
     unsigned long bcap = 256;          // BCAP: most particle blocks per chain
     double lStart = 0.0;               // LSTART: NPT initial length (0 = N, the reference's)
     int device = 0;                    // DEVICE

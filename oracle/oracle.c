@@ -646,6 +646,47 @@ void orc_observables_x(const orc_spec *s, const double *x, double L, double *obs
     }
 }
 
+/* ----------------------------------------------------------- histograms */
+
+static double fold_x(double x, double L, int per)
+{
+    return per ? x - L * floor((x + L / 2.0) / L) : x;
+}
+
+/* One sample of the density and pair-separation histograms, all pairs
+ * (fgrho, src/jmmMCState.cpp:1069-1128): rho bin floor(x/rbw + rhonb/2);
+ * pair bin floor(d/gbw) for d < gnb*gbw counted in the segment
+ * floor(x/gsw + gns/2) of each member.  Periodic: positions folded into
+ * [-L/2, L/2) and each pair at its separation below L/2, the separation
+ * across the end formed as (x_j + L) - x_i.  Counts are added to rho/g. */
+void orc_hist_x(const orc_spec *s, const double *x, double L, uint32_t rhonb, double rbw,
+                uint32_t gns, double gsw, uint32_t gnb, double gbw, uint64_t *rho, uint64_t *g)
+{
+    const int per = s->boundary == ORC_PERIODIC;
+    const int64_t N = (int64_t)s->N;
+    const double gmax = gnb * gbw;
+    for (int64_t i = 0; i < N; ++i) {
+        const double fi = fold_x(x[i], L, per);
+        const int64_t rb = (int64_t)floor(fi / rbw + rhonb / 2.0);
+        if (rb >= 0 && rb < (int64_t)rhonb) rho[rb]++;
+        if (!gns || !gnb) continue;
+        const int64_t si = (int64_t)floor(fi / gsw + gns / 2.0);
+        for (int64_t j = 0; j < N; ++j) {
+            if (j == i) continue;
+            double d;
+            if (j > i) d = x[j] - x[i];
+            else if (per) d = (x[j] + L) - x[i];
+            else continue;
+            if (!(d < gmax) || (per && !(d < L / 2.0))) continue;
+            const int64_t gb = (int64_t)floor(d / gbw);
+            if (gb < 0 || gb >= (int64_t)gnb) continue;
+            const int64_t sj = (int64_t)floor(fold_x(x[j], L, per) / gsw + gns / 2.0);
+            if (si >= 0 && si < (int64_t)gns) g[si * gnb + gb]++;
+            if (sj >= 0 && sj < (int64_t)gns) g[sj * gnb + gb]++;
+        }
+    }
+}
+
 /* --------------------------------------------------------------- sweeps */
 
 static void block_of(uint64_t N, uint64_t B, uint64_t b, uint64_t *start, uint64_t *size)

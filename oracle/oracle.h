@@ -41,6 +41,9 @@
  *                      errors; incremental energy == full recompute
  *   orc_observables    central finite differences in theta and x
  *   orc_det_exp        libm exp to 2 ulp over the move range
+ *   orc_hist_x         hand-binned three- and four-particle cases (walls
+ *                      and periodic, including the pair across the end) and
+ *                      count identities (sum of rho = particles in range)
  *   orc_u32, orc_u53   end points and exact values; the trial stream
  *                      (particle pair per Philox block) recomputed from
  *                      orc_philox4x32_10 and replayed through orc_trial_with
@@ -89,6 +92,8 @@ typedef struct orc_chain orc_chain;
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_u53(uint32_t a, uint32_t b);
 double orc_u32(uint32_t w);
+void orc_hist_x(const orc_spec *s, const double *x, double L, uint32_t rhonb, double rbw,
+                uint32_t gns, double gsw, uint32_t gnb, double gbw, uint64_t *rho, uint64_t *g);
 double orc_det_exp(double x);
 
 /* potentials: out[0]=E, out[1]=W=-d*dphi/dd, out[2]=dE/dtheta0, out[3]=dE/dtheta1,

@@ -57,6 +57,9 @@ def lib():
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_u53.argtypes = [C.c_uint32, C.c_uint32]
         L.orc_u53.restype = C.c_double
+        L.orc_hist_x.argtypes = [C.POINTER(Spec), dp, C.c_double, C.c_uint32, C.c_double,
+                                 C.c_uint32, C.c_double, C.c_uint32, C.c_double,
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.orc_u32.argtypes = [C.c_uint32]
         L.orc_u32.restype = C.c_double
         L.orc_det_exp.argtypes = [C.c_double]
@@ -238,6 +241,17 @@ def run_batch(specs, n_sweeps, n_threads, allpairs=False):
 
 def u53(a, b):
     return lib().orc_u53(a, b)
+
+
+def hist_x(spec: Spec, x, L, rhonb, rbw, gns, gsw, gnb, gbw):
+    """(rho[rhonb], g[gns, gnb]) counts of one configuration."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rho = np.zeros(rhonb, dtype=np.uint64)
+    g = np.zeros(max(gns * gnb, 1), dtype=np.uint64)
+    u64p = C.POINTER(C.c_uint64)
+    lib().orc_hist_x(C.byref(spec), _dp(x), L, rhonb, rbw, gns, gsw, gnb, gbw,
+                     rho.ctypes.data_as(u64p), g.ctypes.data_as(u64p))
+    return rho, g[: gns * gnb].reshape(gns, gnb)
 
 
 def u32(w):

@@ -174,3 +174,52 @@ def test_driver_team_sizes_give_identical_output(gpu, driver, tmp_path):
         outs.append(((d / "thermo.dat.mcs").read_text(), (d / "config.dat.mcs").read_text()))
     assert outs[0] == outs[1]
     assert len(outs[0][0].splitlines()) == 1 + 1 + 20
+
+
+@pytest.mark.gpu
+def test_driver_writes_density_and_pair_histograms(gpu, oracle, driver, tmp_path):
+    """RHOPI/GPI output (printRho/printG, src/jmmMCState.cpp:1021-1062):
+    rho.dat.mcs rows integrate to N (every particle inside the binned range),
+    the step-0 row is the lattice, and g<s>.dat.mcs rows equal the oracle's
+    binning of the same sweeps (walls, FP64 form, one sample per sweep)."""
+    import pyoracle
+    from jmmonedmc_amd.engine import Chain
+    from jmmonedmc_amd import _native as nat
+    from helpers import lj
+    inp = tmp_path / "INPUT"
+    N = 40
+    inp.write_text(f"ENSEMBLE NLT\nN {N}\nL 80\nT 1.0\nPOT LJcut 3.0\nMAXSTEP 0.5\n"
+                   f"NUMSTEPS {N * 50}\nCPI {N * 50}\nTPI {N * 10}\nRHOPI {N * 10}\nGPI {N * 25}\n"
+                   "RBW 2.0\nRHONB 40\nGSW 40.0\nGNS 2\nGBW 0.5\nGNB 12\nSEED 5\nENGCHECK 0\n"
+                   "BOUNDARY WALLS\nFORM BATCHED_FP64\nHISTEVERY 1\n")
+    print("[gpu] driver histograms", flush=True)
+    r = subprocess.run([driver, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = [l.split() for l in (tmp_path / "rho.dat.mcs").read_text().splitlines()]
+    assert [int(row[0]) for row in rows] == [0, 400, 800, 1200, 1600, 2000]
+    for row in rows:
+        vals = np.array([float(v) for v in row[1:]])
+        assert len(vals) == 40
+        assert abs(vals.sum() * 2.0 - N) < 1e-5 * N
+    lattice = ((np.arange(N) + 0.5) / N - 0.5) * 80.0
+    want0 = np.bincount(np.floor(lattice / 2.0 + 20).astype(int), minlength=40) / 2.0
+    assert np.allclose([float(v) for v in rows[0][1:]], want0)
+    # g files: replay 50 sweeps in the oracle, one sample per sweep
+    ch = Chain(N=N, L0=80.0, T=1.0, seed=5, boundary=nat.WALLS, **lj(shifted=0))
+    orc = pyoracle.Chain(oracle_spec(pyoracle, ch))
+
+    def sample():
+        x, L, _, _ = orc.state()
+        return pyoracle.hist_x(orc.spec, x, L, 40, 2.0, 2, 40.0, 12, 0.5)[1].astype(float)
+
+    g0 = sample()                          # printed at step 0 from one sample
+    acc = [np.zeros((2, 12)), np.zeros((2, 12))]
+    for sw in range(50):
+        orc.sweeps(1)
+        acc[0 if sw < 25 else 1] += sample()
+    for s in range(2):
+        grows = [l.split() for l in (tmp_path / f"g{s}.dat.mcs").read_text().splitlines()]
+        assert [int(g[0]) for g in grows] == [0, 1000, 2000]
+        for row, want in zip(grows, (g0[s] / 1, acc[0][s] / 25, acc[1][s] / 25)):
+            got = np.array([float(v) for v in row[1:]])
+            assert np.allclose(got, want / 40.0 / 0.5, rtol=1e-7, atol=0), (s, row[0])

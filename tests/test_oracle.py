@@ -413,3 +413,41 @@ def test_observables_by_finite_differences(oracle):
             up = oracle.brute_energy_x(s, x * (1 + e), 40.0 * (1 + e))
             dn = oracle.brute_energy_x(s, x * (1 - e), 40.0 * (1 - e))
             assert abs(obs[1] + (up - dn) / (2 * e)) <= 1e-6 * max(1, abs(obs[1]))
+
+
+def _hist_spec(oracle, N, L, boundary):
+    from jmmonedmc_amd.engine import Chain
+    from jmmonedmc_amd import _native as nat
+    from helpers import lj, oracle_spec
+    return oracle_spec(oracle, Chain(N=N, L0=L, boundary=boundary, **lj()))
+
+
+def test_hist_hand_binned_walls(oracle):
+    """Three particles between walls (fgrho, src/jmmMCState.cpp:1069-1128):
+    rho bins floor(x/rbw + rhonb/2), pair bins floor(d/gbw) in each member's
+    segment floor(x/gsw + gns/2); binned by hand."""
+    from jmmonedmc_amd import _native as nat
+    s = _hist_spec(oracle, 3, 10.0, nat.WALLS)
+    rho, g = oracle.hist_x(s, [-2.5, 0.2, 1.3], 10.0, 10, 1.0, 2, 5.0, 4, 1.0)
+    assert rho.tolist() == [0, 0, 1, 0, 0, 1, 1, 0, 0, 0]
+    # pairs: (0,1) d=2.7 bin 2, (0,2) d=3.8 bin 3, (1,2) d=1.1 bin 1;
+    # segments: x=-2.5 -> 0, x=0.2 -> 1, x=1.3 -> 1
+    assert g.tolist() == [[0, 0, 1, 1], [0, 2, 1, 1]]
+
+
+def test_hist_hand_binned_periodic(oracle):
+    """Four particles on a ring of length 10: every pair once at its
+    separation below L/2, the pairs across the end included."""
+    from jmmonedmc_amd import _native as nat
+    s = _hist_spec(oracle, 4, 10.0, nat.PERIODIC)
+    x = [-4.75, -1.0, 2.0, 4.5]
+    rho, g = oracle.hist_x(s, x, 10.0, 10, 1.0, 2, 5.0, 5, 1.0)
+    assert rho.tolist() == [1, 0, 0, 0, 1, 0, 0, 1, 0, 1]
+    # (0,1) 3.75 -> 3, (1,2) 3.0 -> 3, (2,3) 2.5 -> 2, across the end:
+    # (3,0) 0.75 -> 0, (3,1) 4.5 -> 4, (2,0) 3.25 -> 3; segments 0,0,1,1
+    assert g.tolist() == [[1, 0, 0, 4, 1], [1, 0, 2, 2, 1]]
+    # a rigid shift that carries particles past +L/2: positions fold back,
+    # separations and so the segment-summed pair counts are unchanged
+    rho2, g2 = oracle.hist_x(s, [v + 3.0 for v in x], 10.0, 10, 1.0, 2, 5.0, 5, 1.0)
+    assert int(rho2.sum()) == 4
+    assert g2.sum(axis=0).tolist() == g.sum(axis=0).tolist()
