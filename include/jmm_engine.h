@@ -186,6 +186,14 @@ int jmm_get_states(jmm_engine *e, double *x, double *L, double *U);
  * Synchronous. */
 int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t sweep);
 
+/* Step sizes between sweeps, for step-size adaptation (the reference's
+ * maxDisAdjust/maxDVAdjust, src/jmmMCState.cpp:2100-2139, rescale maxStep and
+ * maxdl in place): max_step[c] and max_dv[c] per chain, NULL leaves that kind
+ * unchanged.  max_step may not exceed the chain's value at creation (the
+ * block schedule and the periodic length bound are built for it).
+ * Synchronous. */
+int jmm_set_steps(jmm_engine *e, const double *max_step, const double *max_dv);
+
 /* Fault injection for the drift guard: adds delta to chain c's running
  * energy (the next recompute must detect and repair it).  Synchronous. */
 int jmm_perturb_energy(jmm_engine *e, int chain, double delta);

@@ -23,7 +23,7 @@ EXPORTED = [
     "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples",
     "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
-    "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read",
+    "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read", "jmm_set_steps",
 ]
 
 
@@ -104,6 +104,7 @@ def lib(path: str = LIB):
     L.jmm_hist_configure.argtypes = [vp, C.POINTER(HistSpec), C.c_uint32]
     L.jmm_hist_read.argtypes = [vp, u64p, u64p, u64p, C.c_int]
     L.jmm_hist_sample.argtypes = [vp]
+    L.jmm_set_steps.argtypes = [vp, dp, dp]
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name == "jmm_last_error" else C.c_int
     _lib = L

@@ -271,6 +271,7 @@ struct jmm_engine {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     uint64_t ev_sweeps = 0;
     uint64_t sweeps_run = 0;            // sweeps launched since creation
+    std::vector<double> step_cap;       // max_step at creation, per internal chain
     // rho(x)/g(x) sampling (jmm_hist_configure)
     uint32_t hist_every = 0;
     jmm_hist_spec hs{};
@@ -551,6 +552,8 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     if (int rc = dalloc(&e->d_sA, (size_t)e->cap * n_chains)) return bail(rc);
     if (o.log_accepts)
         if (int rc = dalloc(&e->d_log, (size_t)e->cap * e->total_N)) return bail(rc);
+    e->step_cap.resize(n_chains);
+    for (int k = 0; k < n_chains; ++k) e->step_cap[k] = e->hch[k].max_step;
     HIPCHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_states, hst.data(), sizeof(DevState) * n_chains, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
@@ -894,6 +897,34 @@ int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t 
     st.sweep = sweep;
     HIPCHK(hipMemcpy(e->d_x + d.x_off, x, sizeof(double) * s.N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_states + k, &st, sizeof st, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int jmm_set_steps(jmm_engine *e, const double *max_step, const double *max_dv)
+{
+    if (!e) return fail(JMM_EINVAL, "jmm_set_steps: null engine");
+    HIPCHK(hipSetDevice(e->device));
+    for (int c = 0; c < e->n; ++c) {
+        const uint32_t k = e->int_of_ext[c];
+        if (max_step && (!(max_step[c] >= 0.0) || max_step[c] > e->step_cap[k]))
+            return fail(JMM_EINVAL, "jmm_set_steps: chain %d: max_step %g outside [0, %g] (the creation value)",
+                        c, max_step ? max_step[c] : 0.0, e->step_cap[k]);
+        if (max_dv && !(max_dv[c] >= 0.0 && std::isfinite(max_dv[c])))
+            return fail(JMM_EINVAL, "jmm_set_steps: chain %d: bad max_dv", c);
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int c = 0; c < e->n; ++c) {
+        const uint32_t k = e->int_of_ext[c];
+        if (max_step) {
+            e->hch[k].max_step = max_step[c];
+            e->specs[c].max_step = max_step[c];
+        }
+        if (max_dv) {
+            e->hch[k].max_dv = max_dv[c];
+            e->specs[c].max_dv = max_dv[c];
+        }
+    }
+    HIPCHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * e->n, hipMemcpyHostToDevice));
     return 0;
 }
 

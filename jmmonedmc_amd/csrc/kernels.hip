@@ -621,7 +621,8 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
     // boundary, ensemble, step sizes, window): read it once per bundle.
     const DevChain &sh = a.chains[__builtin_amdgcn_readfirstlane(a.slots[bd.slot0].chain)];
     const Pot q = sh.pot;
-    const double gmin = sh.gmin, max_step = sh.max_step;
+    const double gmin = sh.gmin;
+    const double max_step = ch.max_step;  // per chain: step adaptation may change it
     // per-chain fields as values: read once, not reloaded after every store
     // of the loops below (the compiler cannot rule out aliasing)
     const double beta = ch.beta;

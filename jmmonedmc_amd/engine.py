@@ -154,6 +154,13 @@ class Engine:
         x = np.ascontiguousarray(x, dtype=np.float64)
         check(lib().jmm_set_state(self.h, chain, _dp(x), L, sweep), "jmm_set_state")
 
+    def set_steps(self, max_step=None, max_dv=None):
+        """Per-chain displacement and box-move half-widths (None = unchanged)."""
+        ms = None if max_step is None else np.ascontiguousarray(max_step, dtype=np.float64)
+        mv = None if max_dv is None else np.ascontiguousarray(max_dv, dtype=np.float64)
+        check(lib().jmm_set_steps(self.h, _dp(ms) if ms is not None else None,
+                                  _dp(mv) if mv is not None else None), "jmm_set_steps")
+
     def perturb_energy(self, chain: int, delta: float):
         check(lib().jmm_perturb_energy(self.h, chain, delta), "jmm_perturb_energy")
 

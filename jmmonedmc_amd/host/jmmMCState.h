@@ -37,8 +37,8 @@ int isMaxDisAdjust(struct MCState *mcs);        // :1870
 int isMaxDVAdjust(struct MCState *mcs);         // :1883
 int updateThermo(struct MCState *mcs);          // :1941
 int ECheck(struct MCState *mcs);                // :1965
-int maxDisAdjust(struct MCState *mcs);          // :2100 (fixed steps: no-op)
-int maxDVAdjust(struct MCState *mcs);           // :2120 (fixed steps: no-op)
+int maxDisAdjust(struct MCState *mcs);          // :2100 Swendsen step adaptation
+int maxDVAdjust(struct MCState *mcs);           // :2120 Swendsen box-step adaptation
 int relaxVolume(struct MCState *mcs);           // :2396 (not part of this engine)
 unsigned long getStepNum(struct MCState *mcs);  // :2777
 int getRelaxFlag(struct MCState *mcs);

@@ -53,6 +53,9 @@ struct MCState {
     std::vector<FILE *> gf;
     std::vector<uint64_t> rhoAcc, gAcc, tmpR, tmpG;
     uint64_t nsRho = 0, nsG = 0;
+    // step-size adaptation (DADJ/VADJ)
+    std::vector<double> maxStep, maxDv;
+    uint64_t vTrialsAtAdjust = 0;
 };
 
 static void die_on(int rc, const char *what)
@@ -105,7 +108,11 @@ static jmm_chain_spec spec_from_input(const MCInput &inp, unsigned long k, bool 
     s.P = inp.P;
     // NPT starts at l = N (src/jmmMCState.cpp:549-555) unless LSTART is given
     s.L0 = npt ? (inp.lStart > 0 ? inp.lStart : (double)inp.N) : inp.L;
-    s.max_step = inp.maxStep;
+    // with DADJ adaptation the step may grow to the reference's clamp 0.5
+    // (src/jmmMCState.cpp:2106-2111): the schedule is built for the larger
+    // of the two and the INPUT value is set right after creation
+    const bool adapt = inp.mdai && inp.mdai < inp.ns;
+    s.max_step = adapt ? std::max(inp.maxStep, 0.5) : inp.maxStep;
     s.max_dv = inp.maxdl;
     s.seed = inp.seed + k;
     s.chain_id = (uint32_t)k;
@@ -199,6 +206,12 @@ struct MCState *setupMCS(struct MCInput inp)
         delete m;
         return nullptr;
     }
+    m->maxStep.assign(m->specs.size(), inp.maxStep);
+    m->maxDv.assign(m->specs.size(), inp.maxdl);
+    if (inp.mdai && inp.mdai < inp.ns) {
+        rc = jmm_set_steps(m->eng, m->maxStep.data(), nullptr);
+        if (rc) die_on(rc, "jmm_set_steps");
+    }
     printMCP(m);
     if ((inp.rhopi && inp.rhonb && inp.rbw > 0.0) || (inp.gpi && inp.gns > 0 && inp.gnb && inp.gbw > 0.0)) {
         jmm_hist_spec hs{};
@@ -288,8 +301,6 @@ struct MCState *setupMCS(struct MCInput inp)
         m->cf = std::fopen("config.dat.mcs", "a");
         m->tf = std::fopen("thermo.dat.mcs", "a");
     }
-    if (inp.mdai && inp.mdai < inp.ns)
-        std::printf("NOTE: DADJ/VADJ step-size adaptation is not run; MAXSTEP and MAXDV stay fixed.\n");
     return m;
 }
 
@@ -505,8 +516,56 @@ int ECheck(struct MCState *m)
     return result;
 }
 
-int maxDisAdjust(struct MCState *) { return 0; }
-int maxDVAdjust(struct MCState *) { return 0; }
+// Swendsen's rule (Physics Procedia 15 (2011) 81): step' = step *
+// ln(a P_ideal + b) / ln(a P + b), a = 0.672924, b = 0.0644284, P the
+// cumulative acceptance ratio.  The reference writes the denominator as
+// ln(a (P + b)) (src/jmmMCState.cpp:2105, :2128); the published form is used.
+static double swendsen(double step, double ratio)
+{
+    const double a = 0.672924, b = 0.0644284;
+    return step * std::log(a * 0.5 + b) / std::log(a * ratio + b);
+}
+
+// src/jmmMCState.cpp:2100-2115: clamp [0.002, 0.5].
+int maxDisAdjust(struct MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    std::vector<jmm_counters> c(m->specs.size());
+    if (int rc = jmm_get_counters(m->eng, -1, c.data())) return rc;
+    for (size_t k = 0; k < c.size(); ++k) {
+        if (!c[k].trials) continue;
+        const double ratio = (double)c[k].acc_d / (double)c[k].trials;
+        m->maxStep[k] = std::min(0.5, std::max(0.002, swendsen(m->maxStep[k], ratio)));
+    }
+    if (int rc = jmm_set_steps(m->eng, m->maxStep.data(), nullptr)) return rc;
+    std::printf("Step: %lu  Updating max Step...dAcc: %lu,%lu   new maxStep: %.5G\n", m->sn,
+                (unsigned long)c[0].acc_d, (unsigned long)(c[0].trials - c[0].acc_d), m->maxStep[0]);
+    return 0;
+}
+
+// src/jmmMCState.cpp:2120-2139: only after new box-length trials; clamp to
+// [0.002 N, 0.1 N] for the linear move (the reference sets 0.5 N above 0.1 N,
+// :2132-2133; the clamp is kept at 0.1 N), [0.002, 0.1] in ln L for the log
+// move.
+int maxDVAdjust(struct MCState *m)
+{
+    if (int rc = flush(m)) return rc;
+    std::vector<jmm_counters> c(m->specs.size());
+    if (int rc = jmm_get_counters(m->eng, -1, c.data())) return rc;
+    if (c[0].vol_trials <= m->vTrialsAtAdjust) return 0;
+    m->vTrialsAtAdjust = c[0].vol_trials;
+    const bool lg = m->inp.volMove == "LOG";
+    const double lo = lg ? 0.002 : 0.002 * m->N, hi = lg ? 0.1 : 0.10 * m->N;
+    for (size_t k = 0; k < c.size(); ++k) {
+        if (!c[k].vol_trials) continue;
+        const double ratio = (double)c[k].acc_v / (double)c[k].vol_trials;
+        m->maxDv[k] = std::min(hi, std::max(lo, swendsen(m->maxDv[k], ratio)));
+    }
+    if (int rc = jmm_set_steps(m->eng, nullptr, m->maxDv.data())) return rc;
+    std::printf("Step: %lu  Updating max vol.ch....vAcc: %lu,%lu   new maxStep: %.5G\n", m->sn,
+                (unsigned long)c[0].acc_v, (unsigned long)(c[0].vol_trials - c[0].acc_v), m->maxDv[0]);
+    return 0;
+}
 
 int relaxVolume(struct MCState *)
 {

@@ -19,7 +19,7 @@ struct MCInput {
     int relaxFlag = 0;                 // RELAX
     unsigned long cpi = 0, tpi = 0;    // CPI, TPI (0 = never)
     unsigned long eci = 0;             // ENGCHECK (trials, 0 = never)
-    unsigned long mdai = 0, mvai = 0;  // DADJ, VADJ (step-size adaptation; not run)
+    unsigned long mdai = 0, mvai = 0;  // DADJ, VADJ (step-size adaptation intervals)
     unsigned long gpi = 0, rhopi = 0;  // GPI, RHOPI
     unsigned long gnb = 0, rhonb = 0;  // GNB, RHONB
     unsigned long seed = 1;            // SEED

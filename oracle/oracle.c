@@ -299,6 +299,8 @@ struct orc_chain {
     orc_spec s;
     double *x;
     double L, U, beta, shift, gmin;
+    double step_cap; /* max_step at creation: the schedule and the periodic
+                        length bound are built for it */
     uint64_t sweep, B, M_min, w;
     uint64_t cnt[8]; /* trials, acc_d, wall_rej, hc_rej, pair_terms, vol_trials, acc_v, recomputes */
     double max_drift;
@@ -326,12 +328,25 @@ orc_chain *orc_chain_new(const orc_spec *s)
     double sh[9];
     c->shift = shift_terms(s, sh);
     c->gmin = gmin_of(s);
+    c->step_cap = s->max_step;
     orc_schedule(s, &c->B, &c->M_min, &c->w);
     /* lattice start, src/jmmMCState.cpp:559-562 */
     for (uint64_t i = 0; i < s->N; ++i)
         c->x[i] = ((i + 0.5) / s->N - 0.5) * c->L;
     c->U = orc_full_energy(c);
     return c;
+}
+
+/* New step sizes between sweeps (the reference's maxDisAdjust/maxDVAdjust,
+ * src/jmmMCState.cpp:2100-2139, change maxStep/maxdl in place); the block
+ * schedule stays the one built at creation, so max_step may not exceed the
+ * creation value.  Returns 0 on success, -1 if it does. */
+int orc_chain_set_steps(orc_chain *c, double max_step, double max_dv)
+{
+    if (!(max_step >= 0.0) || max_step > c->step_cap || !(max_dv >= 0.0)) return -1;
+    c->s.max_step = max_step;
+    c->s.max_dv = max_dv;
+    return 0;
 }
 
 void orc_chain_free(orc_chain *c)
@@ -533,7 +548,7 @@ int orc_volume_trial_with(orc_chain *c, double dv, double u, double *dU_out)
         dL = Lp - c->L;
     }
     if (!(Lp > 0.0)) return 0;
-    if (s->boundary == ORC_PERIODIC && !(Lp > 2.0 * (s->cutoff + s->max_step))) return 0;
+    if (s->boundary == ORC_PERIODIC && !(Lp > 2.0 * (s->cutoff + c->step_cap))) return 0;
     double dU;
     if (!scaled_dU(c, sc, Lp, &dU)) return 0;
     if (dU_out) *dU_out = dU;

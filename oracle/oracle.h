@@ -110,6 +110,7 @@ void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w);
 /* chain lifecycle */
 orc_chain *orc_chain_new(const orc_spec *s);
 void orc_chain_free(orc_chain *c);
+int orc_chain_set_steps(orc_chain *c, double max_step, double max_dv);
 void orc_chain_get(const orc_chain *c, double *x, double *L, double *U, uint64_t *sweep);
 void orc_chain_set(orc_chain *c, const double *x, double L, uint64_t sweep);
 uint64_t orc_chain_counter(const orc_chain *c, int which);

@@ -60,6 +60,7 @@ def lib():
         L.orc_hist_x.argtypes = [C.POINTER(Spec), dp, C.c_double, C.c_uint32, C.c_double,
                                  C.c_uint32, C.c_double, C.c_uint32, C.c_double,
                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.orc_chain_set_steps.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_u32.argtypes = [C.c_uint32]
         L.orc_u32.restype = C.c_double
         L.orc_det_exp.argtypes = [C.c_double]
@@ -187,6 +188,10 @@ class Chain:
     def set_state(self, x, L, sweep=0):
         x = np.ascontiguousarray(x, dtype=np.float64)
         lib().orc_chain_set(self.h, _dp(x), L, sweep)
+
+    def set_steps(self, max_step, max_dv):
+        if lib().orc_chain_set_steps(self.h, max_step, max_dv) != 0:
+            raise ValueError("max_step above the creation value")
 
     def counter(self, name):
         return lib().orc_chain_counter(self.h, self.COUNTERS[name])

@@ -2,6 +2,7 @@
 src/testInput.c:15-63, and the four INPUT files it ships), the C ABI's
 exports, and (GPU) the driver replayed against the oracle.
 """
+import math
 import os
 import re
 import subprocess
@@ -223,3 +224,29 @@ def test_driver_writes_density_and_pair_histograms(gpu, oracle, driver, tmp_path
         for row, want in zip(grows, (g0[s] / 1, acc[0][s] / 25, acc[1][s] / 25)):
             got = np.array([float(v) for v in row[1:]])
             assert np.allclose(got, want / 40.0 / 0.5, rtol=1e-7, atol=0), (s, row[0])
+
+
+@pytest.mark.gpu
+def test_driver_step_adaptation(gpu, driver, tmp_path):
+    """DADJ/VADJ (maxDisAdjust/maxDVAdjust, src/jmmMCState.cpp:2100-2139):
+    every adjustment follows Swendsen's rule from the cumulative acceptance
+    counts it prints, within the reference's clamps."""
+    inp = tmp_path / "INPUT"
+    N = 60
+    inp.write_text(f"ENSEMBLE NPT\nN {N}\nP 1.0\nT 0.8\nPOT LJshift 3.0\nMAXSTEP 0.05\nMAXDV 1.0\n"
+                   f"NUMSTEPS {(N + 1) * 400}\nCPI {(N + 1) * 400}\nTPI {(N + 1) * 100}\n"
+                   f"DADJ {(N + 1) * 50}\nVADJ {(N + 1) * 50}\nSEED 11\nBOUNDARY WALLS\nLSTART 90\n")
+    print("[gpu] driver step adaptation", flush=True)
+    r = subprocess.run([driver, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    a, b = 0.672924, 0.0644284
+    for key, start, lo, hi in (("max Step", 0.05, 0.002, 0.5), ("max vol.ch", 1.0, 0.002 * N, 0.1 * N)):
+        lines = re.findall(r"Updating " + re.escape(key) + r"\.\.\.\.?\w+: (\d+),(\d+)\s+new maxStep: (\S+)",
+                           r.stdout)
+        assert len(lines) == 8, (key, lines)
+        prev = start
+        for acc, rej, new in lines:
+            ratio = int(acc) / (int(acc) + int(rej))
+            want = min(hi, max(lo, prev * math.log(a * 0.5 + b) / math.log(a * ratio + b)))
+            assert float(new) == pytest.approx(want, rel=2e-4), (key, prev, ratio, new)
+            prev = float(new)

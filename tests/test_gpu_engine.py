@@ -162,3 +162,35 @@ def test_histograms_equal_oracle_counts(gpu, oracle, case):
         assert ns2 == 4 and np.array_equal(rho2, rho)
         _, _, ns3 = e.hist_read()
         assert ns3 == 0
+
+
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64"])
+def test_step_sizes_changed_between_sweeps_replay_oracle(gpu, oracle, form):
+    """jmm_set_steps (step adaptation support): new displacement and box-move
+    half-widths after 10 sweeps; the next 15 sweeps equal the oracle's with
+    the same change, move for move.  A step above the creation value is
+    refused (the schedule is built for it)."""
+    print(f"[gpu] set_steps {form}", flush=True)
+    chains = CASES["npt_periodic_log"]()
+    n = len(chains)
+    new_step = [0.2 + 0.05 * c for c in range(n)]
+    new_dv = [0.003 * (c + 1) for c in range(n)]
+    with Engine(chains, form=form, sample_capacity=32, block_sweeps=8, log_accepts=True) as e:
+        e.run(10)
+        e.read_accept_log(32)
+        e.read_samples(32)
+        e.set_steps(new_step, new_dv)
+        with pytest.raises(JmmError, match="creation value"):
+            e.set_steps([ch.max_step * 1.5 for ch in chains])
+        e.run(15)
+        logs = e.read_accept_log(32)
+        for c, ch in enumerate(chains):
+            orc = oracle.Chain(oracle_spec(oracle, ch))
+            orc.sweeps(10)
+            orc.set_steps(new_step[c], new_dv[c])
+            _, olog = orc.sweeps(15, log=True)
+            x, L, U, _ = e.state(c)
+            ox, oL, oU, _ = orc.state()
+            assert np.array_equal(logs[c].ravel(), olog), c
+            assert np.array_equal(x, ox) and L == oL
+            assert rel(U, oU) <= 1e-10
