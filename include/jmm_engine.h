@@ -142,7 +142,7 @@ int jmm_schedule(const jmm_chain_spec *spec, uint64_t *B, uint64_t *M_min, uint6
  * potential select :291-399, ensemble :401-425, lattice :549-562, RNG
  * :779-781) and the first-step fad energy (src/Main.cpp:66-68), for n_chains
  * chains at once.  Uploads the lattice start and computes each chain's FP64
- * energy.  *out receives the handle. */
+ * energy on the GPU.  *out receives the handle. */
 int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts *opts,
                jmm_engine **out);
 
@@ -193,6 +193,18 @@ int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t 
  * block schedule and the periodic length bound are built for it).
  * Synchronous. */
 int jmm_set_steps(jmm_engine *e, const double *max_step, const double *max_dv);
+
+/* FP64 energy of each chain's configuration rescaled to length L_try[c]
+ * (x -> x L_try/L), the state unchanged (calculateEnergyOfTrialVolumeChange,
+ * src/jmmMCState.cpp:2783-2827, used by relaxVolume :2396).
+ * U_out[n_chains].  Synchronous. */
+int jmm_scaled_energy(jmm_engine *e, const double *L_try, double *U_out);
+
+/* Rescales every chain to length L_new[c] (moveVolume,
+ * src/jmmMCState.cpp:2831-2916): x -> x L_new/L, energy recomputed in FP64.
+ * Refused if a pair would fall inside the hard core, a particle outside the
+ * walls, or a periodic length below 2 (cutoff + max_step).  Synchronous. */
+int jmm_set_lengths(jmm_engine *e, const double *L_new);
 
 /* Fault injection for the drift guard: adds delta to chain c's running
  * energy (the next recompute must detect and repair it).  Synchronous. */

@@ -24,6 +24,7 @@ EXPORTED = [
     "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
     "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read", "jmm_set_steps",
+    "jmm_scaled_energy", "jmm_set_lengths",
 ]
 
 
@@ -105,6 +106,8 @@ def lib(path: str = LIB):
     L.jmm_hist_read.argtypes = [vp, u64p, u64p, u64p, C.c_int]
     L.jmm_hist_sample.argtypes = [vp]
     L.jmm_set_steps.argtypes = [vp, dp, dp]
+    L.jmm_scaled_energy.argtypes = [vp, dp, dp]
+    L.jmm_set_lengths.argtypes = [vp, dp]
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name == "jmm_last_error" else C.c_int
     _lib = L

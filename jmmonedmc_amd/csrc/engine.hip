@@ -18,6 +18,8 @@
 
 using namespace jmm;
 
+static int refresh_energies(jmm_engine *e, int only);
+
 namespace {
 
 thread_local std::string g_err;
@@ -143,38 +145,6 @@ Pot make_pot(const jmm_chain_spec &s)
     return q;
 }
 
-// Host FP64 energy in the oracle's order (every pair once, from its left
-// member, nearest partner first): the initial energy (fad at step 0,
-// src/Main.cpp:66-68) and the restart energy.
-template <int KIND>
-double host_energy_t(const Pot &q, const double *x, int64_t N, bool per, double L, uint64_t w)
-{
-    double U = 0.0;
-    for (int64_t i = 0; i < N; ++i)
-        for (uint64_t k = 1; k <= w; ++k) {
-            const int64_t j = i + (int64_t)k;
-            double xj;
-            if (j < N) xj = x[j];
-            else if (per) xj = x[j - N] + L;
-            else break;
-            const double d = xj - x[i];
-            if (d > q.rc) break;
-            U += phi64<KIND>(q, d);
-        }
-    return U;
-}
-
-double host_energy(const jmm_chain_spec &s, const Pot &q, const double *x, double L, uint64_t w)
-{
-    const bool per = s.boundary == JMM_PERIODIC;
-    switch (s.potential) {
-    case JMM_POT_LJ: return host_energy_t<POT_LJ>(q, x, (int64_t)s.N, per, L, w);
-    case JMM_POT_HARMONIC: return host_energy_t<POT_HARMONIC>(q, x, (int64_t)s.N, per, L, w);
-    case JMM_POT_HRSW: return host_energy_t<POT_HRSW>(q, x, (int64_t)s.N, per, L, w);
-    default: return host_energy_t<POT_YUKAWA>(q, x, (int64_t)s.N, per, L, w);
-    }
-}
-
 // Shifted-form values at the cutoff for the observables (E, dE/dtheta0,
 // dE/dtheta1), computed with the unshifted pair function.
 void shift_obs(const jmm_chain_spec &s, const Pot &q, double sh[3])
@@ -202,6 +172,7 @@ DevChain make_chain(const jmm_chain_spec &s)
     c.N = (uint32_t)s.N;
     c.B = (uint32_t)B;
     c.w = (uint32_t)w;
+    c.nbn_lim = s.nbn >= 0 ? (uint32_t)s.nbn : UINT32_MAX;
     c.smax = (uint32_t)((s.N + B - 1) / B);
     c.wg = c.w > 0 ? c.w : 1;
     c.Sp = c.smax | 1u;
@@ -261,7 +232,7 @@ struct jmm_engine {
     double *d_sU = nullptr, *d_sL = nullptr;
     uint2 *d_sA = nullptr;
     uint8_t *d_log = nullptr;
-    double *d_obs = nullptr, *d_grad = nullptr, *d_Ltmp = nullptr;
+    double *d_obs = nullptr, *d_grad = nullptr, *d_Ltmp = nullptr, *d_Lnew = nullptr;
     uint32_t *d_all = nullptr;
     uint32_t cap = 1024, write_pos = 0, block_sweeps = 100;
     std::vector<Group> grp;
@@ -442,6 +413,7 @@ int jmm_destroy(jmm_engine *e)
     (void)hipFree(e->d_obs);
     (void)hipFree(e->d_grad);
     (void)hipFree(e->d_Ltmp);
+    (void)hipFree(e->d_Lnew);
     (void)hipFree(e->d_all);
     (void)hipFree(e->d_rho);
     (void)hipFree(e->d_g);
@@ -539,7 +511,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         for (uint64_t i = 0; i < s.N; ++i) x[i] = (((double)i + 0.5) / (double)s.N - 0.5) * s.L0;
         DevState st{};
         st.L = s.L0;
-        st.U = host_energy(s, d.pot, x, s.L0, d.w);
+        st.U = 0.0;  // set from the GPU energy kernel below
         hst[k] = st;
         e->hch[k] = d;
         xoff += s.N;
@@ -557,6 +529,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     HIPCHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_states, hst.data(), sizeof(DevState) * n_chains, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
+    if (int rc = refresh_energies(e, -1)) return bail(rc);
     // launch groups: plain form by LDS fit; batched forms by (memory mode,
     // potential), one persistent launch each
     if (e->form == JMM_FORM_PLAIN_FP64) {
@@ -893,11 +866,11 @@ int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t 
     DevState st;
     HIPCHK(hipMemcpy(&st, e->d_states + k, sizeof st, hipMemcpyDeviceToHost));
     st.L = L;
-    st.U = host_energy(s, d.pot, x, L, d.w);
+    st.U = 0.0;
     st.sweep = sweep;
     HIPCHK(hipMemcpy(e->d_x + d.x_off, x, sizeof(double) * s.N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_states + k, &st, sizeof st, hipMemcpyHostToDevice));
-    return 0;
+    return refresh_energies(e, (int)k);
 }
 
 int jmm_set_steps(jmm_engine *e, const double *max_step, const double *max_dv)
@@ -974,7 +947,8 @@ int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
     return 0;
 }
 
-static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32)
+static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32,
+                            const double *Lnew_dev = nullptr)
 {
     HIPCHK(hipSetDevice(e->device));
     if (!e->d_obs)
@@ -995,6 +969,7 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32)
     a.x = e->d_x;
     a.L = e->d_Ltmp;
     a.list = e->d_all;
+    a.Lnew = Lnew_dev;
     a.obs = e->d_obs;
     a.grad = grad ? e->d_grad : nullptr;
     a.n = (uint32_t)e->n;
@@ -1014,6 +989,86 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32)
         }
     }
     return 0;
+}
+
+// FP64 energies of the current configurations (the GPU observables kernel)
+// written into the chain states: every chain (only = -1) or one internal
+// chain.  Plays the role of fad's step-0 energy (src/Main.cpp:66-68).
+static int refresh_energies(jmm_engine *e, int only)
+{
+    std::vector<double> obs((size_t)e->n * 6);
+    if (int rc = observables_impl(e, obs.data(), nullptr, 0)) return rc;
+    std::vector<DevState> st(e->n);
+    HIPCHK(hipMemcpy(st.data(), e->d_states, sizeof(DevState) * e->n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < e->n; ++k)
+        if (only < 0 || k == only) st[k].U = obs[(size_t)e->ext_of_int[k] * 6];
+    HIPCHK(hipMemcpy(e->d_states, st.data(), sizeof(DevState) * e->n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Per-chain lengths (external order) copied to the device in internal order.
+static int upload_lengths(jmm_engine *e, const double *L_ext)
+{
+    if (!e->d_Lnew)
+        if (int rc = dalloc(&e->d_Lnew, (size_t)e->n)) return rc;
+    std::vector<double> h(e->n);
+    for (int k = 0; k < e->n; ++k) h[k] = L_ext[e->ext_of_int[k]];
+    HIPCHK(hipMemcpy(e->d_Lnew, h.data(), sizeof(double) * e->n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int jmm_scaled_energy(jmm_engine *e, const double *L_try, double *U_out)
+{
+    if (!e || !L_try || !U_out) return fail(JMM_EINVAL, "jmm_scaled_energy: null argument");
+    for (int c = 0; c < e->n; ++c)
+        if (!(L_try[c] > 0.0) || !std::isfinite(L_try[c]))
+            return fail(JMM_EINVAL, "jmm_scaled_energy: chain %d: bad length %g", c, L_try[c]);
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (int rc = upload_lengths(e, L_try)) return rc;
+    std::vector<double> obs((size_t)e->n * 6);
+    if (int rc = observables_impl(e, obs.data(), nullptr, 0, e->d_Lnew)) return rc;
+    for (int c = 0; c < e->n; ++c) U_out[c] = obs[(size_t)c * 6];
+    return 0;
+}
+
+int jmm_set_lengths(jmm_engine *e, const double *L_new)
+{
+    if (!e || !L_new) return fail(JMM_EINVAL, "jmm_set_lengths: null argument");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<DevState> st(e->n);
+    HIPCHK(hipMemcpy(st.data(), e->d_states, sizeof(DevState) * e->n, hipMemcpyDeviceToHost));
+    // the scaled configuration must keep the hard core and (periodic) the
+    // length bound the window is built for
+    std::vector<double> x(e->total_N);
+    HIPCHK(hipMemcpy(x.data(), e->d_x, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));
+    for (int c = 0; c < e->n; ++c) {
+        const uint32_t k = e->int_of_ext[c];
+        const DevChain &d = e->hch[k];
+        const bool per = (d.flags & F_PERIODIC) != 0;
+        const double Ln = L_new[c];
+        if (!(Ln > 0.0) || !std::isfinite(Ln) || (per && !(Ln > d.Lmin)))
+            return fail(JMM_EINVAL, "jmm_set_lengths: chain %d: bad length %g", c, Ln);
+        const double sc = Ln / st[k].L;
+        const double *xc = x.data() + d.x_off;
+        double gmin_seen = INFINITY;
+        for (uint64_t i = 0; i + 1 < d.N; ++i) gmin_seen = std::min(gmin_seen, (xc[i + 1] - xc[i]) * sc);
+        if (per && d.N >= 1) gmin_seen = std::min(gmin_seen, (xc[0] + st[k].L - xc[d.N - 1]) * sc);
+        if (!(gmin_seen >= d.gmin) && d.N >= 2)
+            return fail(JMM_EINVAL, "jmm_set_lengths: chain %d: length %g puts a pair inside the hard core", c, Ln);
+        if (!per && d.N >= 1 && (std::fabs(xc[0] * sc) > Ln / 2.0 || std::fabs(xc[d.N - 1] * sc) > Ln / 2.0))
+            return fail(JMM_EINVAL, "jmm_set_lengths: chain %d: positions outside the walls", c);
+    }
+    if (int rc = upload_lengths(e, L_new)) return rc;
+    uint64_t nmax = 1;
+    for (const auto &d : e->hch) nmax = std::max<uint64_t>(nmax, d.N);
+    HIPCHK(launch_rescale(e->d_chains, e->d_states, e->d_x, e->d_Lnew, (uint32_t)e->n,
+                          (uint32_t)std::min<uint64_t>(64, (nmax + 255) / 256), e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int c = 0; c < e->n; ++c) st[e->int_of_ext[c]].L = L_new[c];
+    HIPCHK(hipMemcpy(e->d_states, st.data(), sizeof(DevState) * e->n, hipMemcpyHostToDevice));
+    return refresh_energies(e, -1);
 }
 
 int jmm_observables(jmm_engine *e, double *obs, double *grad)

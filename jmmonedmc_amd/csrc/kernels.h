@@ -36,7 +36,7 @@ struct DevChain {
     uint32_t chain_id, eci, flags, kind;
     uint32_t wg, Sp;  // LDS layout: ghost width max(w, 1), odd block stride >= smax
     uint32_t lds_len; // LDS doubles of the chain's region: 2 wg + B Sp
-    uint32_t pad0;
+    uint32_t nbn_lim;  // NBN partner limit (UINT32_MAX: none)
     double beta, P, max_step, max_dv, gmin, Lmin;
     double sh[3];  // shifted form at the cutoff: E, dE/dtheta0, dE/dtheta1
     Pot pot;
@@ -84,6 +84,7 @@ struct ObsArgs {
     const double *x;
     const double *L;
     const uint32_t *list;    // engine form: chains of this launch
+    const double *Lnew;      // engine form, nullable: evaluate at x * Lnew/L, length Lnew
     double *obs;
     double *grad;
     uint32_t n;
@@ -113,4 +114,6 @@ hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, h
 hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s);
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s);
 hipError_t launch_hist(const HistArgs &a, hipStream_t s);
+hipError_t launch_rescale(const DevChain *chains, const DevState *st, double *x, const double *Lnew,
+                          uint32_t n_chains, uint32_t parts, hipStream_t s);
 }  // namespace jmm

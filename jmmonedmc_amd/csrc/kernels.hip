@@ -1062,24 +1062,43 @@ __global__ void k_begin(Ctrl *c, uint32_t n)
 // One workgroup per configuration.  Pair arithmetic in Real, sums in FP64.
 // obs = (U, W, dU/dtheta0, dU/dtheta1, dW/dtheta0, dW/dtheta1); grad = dU/dx.
 template <typename Real, int KIND>
-__device__ void run_obs(const DevChain &ch, const double *X, double L, double *obs, double *grad)
+__device__ void run_obs(const DevChain &ch, const double *X, double L, double sc, double *obs,
+                        double *grad)
 {
     __shared__ double red[4 * 6];
     const uint32_t tid = threadIdx.x;
     const Pot q = ch.pot;
     const int64_t N = ch.N;
-    const uint32_t w = ch.w;
+    // a compressed configuration (sc < 1) may hold gaps below the hard core:
+    // walk to the cutoff (or the NBN limit) without the window limit
+    const uint32_t w = sc < 1.0 ? min(ch.nbn_lim, (uint32_t)(N > 0 ? N - 1 : 0)) : ch.w;
     const bool per = (ch.flags & F_PERIODIC) != 0;
     const Real sh[3] = {Real(ch.sh[0]), Real(ch.sh[1]), Real(ch.sh[2])};
+    // position j (index possibly past either end on a periodic line) of the
+    // configuration scaled by sc to length L: x_j sc, images x sc -+ L
+    auto pos = [&](int64_t j, double &xj) -> bool {
+        if (j >= 0 && j < N) {
+            xj = X[j] * sc;
+            return true;
+        }
+        if (!per) return false;
+        xj = (j < 0) ? X[j + N] * sc - L : X[j - N] * sc + L;
+        return true;
+    };
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t i = tid; i < (uint32_t)N; i += 256) {
-        const double xi = X[i];
+        const double xi = X[i] * sc;
         double g = 0.0;
         for (uint32_t k = 1; k <= w; ++k) {
             double xj;
-            if (!nbpos(X, (int64_t)i + k, N, per, L, xj)) break;
+            if (!pos((int64_t)i + k, xj)) break;
             const double d = xj - xi;
             if (d > q.rc) break;
+            // a rescaled configuration can overlap hard cores: infinite energy
+            if ((KIND == POT_HRSW && d < q.p0) || (KIND == POT_YUKAWA && d < q.p2)) {
+                acc[0] += INFINITY;
+                continue;
+            }
             if (!inside64<KIND>(q, d)) continue;
             Real o[6];
             pair_obs<KIND, Real>(q, Real(d), sh, o);
@@ -1090,7 +1109,7 @@ __device__ void run_obs(const DevChain &ch, const double *X, double L, double *o
         if (grad) {
             for (uint32_t k = 1; k <= w; ++k) {
                 double xj;
-                if (!nbpos(X, (int64_t)i - k, N, per, L, xj)) break;
+                if (!pos((int64_t)i - k, xj)) break;
                 const double d = xi - xj;
                 if (d > q.rc) break;
                 if (!inside64<KIND>(q, d)) continue;
@@ -1118,6 +1137,7 @@ __global__ void __launch_bounds__(256) k_observables(ObsArgs a)
     const double *X;
     double L, *grad;
     double *obs;
+    double sc = 1.0;
     if (a.standalone) {
         X = a.x + (uint64_t)c * ch.N;
         L = a.L[c];
@@ -1127,14 +1147,18 @@ __global__ void __launch_bounds__(256) k_observables(ObsArgs a)
         const uint32_t ic = a.list[c];
         X = a.x + ch.x_off;
         L = a.L[ic];
+        if (a.Lnew) {
+            sc = a.Lnew[ic] / L;
+            L = a.Lnew[ic];
+        }
         grad = a.grad ? a.grad + ch.x_off : nullptr;
         obs = a.obs + (uint64_t)ic * 6;
     }
     switch (ch.kind) {
-    case POT_LJ: run_obs<Real, POT_LJ>(ch, X, L, obs, grad); break;
-    case POT_HARMONIC: run_obs<Real, POT_HARMONIC>(ch, X, L, obs, grad); break;
-    case POT_HRSW: run_obs<Real, POT_HRSW>(ch, X, L, obs, grad); break;
-    default: run_obs<Real, POT_YUKAWA>(ch, X, L, obs, grad); break;
+    case POT_LJ: run_obs<Real, POT_LJ>(ch, X, L, sc, obs, grad); break;
+    case POT_HARMONIC: run_obs<Real, POT_HARMONIC>(ch, X, L, sc, obs, grad); break;
+    case POT_HRSW: run_obs<Real, POT_HRSW>(ch, X, L, sc, obs, grad); break;
+    default: run_obs<Real, POT_YUKAWA>(ch, X, L, sc, obs, grad); break;
     }
 }
 
@@ -1205,6 +1229,18 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs a)
     }
 }
 
+// Positions of every chain scaled to a new length: x -> x Lnew/L (moveVolume,
+// src/jmmMCState.cpp:2831-2916).  The length itself is set by the host.
+__global__ void __launch_bounds__(256) k_rescale(const DevChain *chains, const DevState *st,
+                                                 double *x, const double *Lnew, uint32_t parts)
+{
+    const uint32_t c = blockIdx.x / parts, part = blockIdx.x % parts;
+    const DevChain &ch = chains[c];
+    const double sc = Lnew[c] / st[c].L;
+    double *X = x + ch.x_off;
+    for (uint32_t i = part * 256 + threadIdx.x; i < ch.N; i += 256 * parts) X[i] = X[i] * sc;
+}
+
 // ------------------------------------------------------------ launchers
 
 hipError_t launch_begin(Ctrl *c, uint32_t n, hipStream_t s)
@@ -1251,6 +1287,13 @@ hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStre
 {
     if (fp32) hipLaunchKernelGGL(k_observables<float>, dim3(grid), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_observables<double>, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescale(const DevChain *chains, const DevState *st, double *x, const double *Lnew,
+                          uint32_t n_chains, uint32_t parts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rescale, dim3(n_chains * parts), dim3(256), 0, s, chains, st, x, Lnew, parts);
     return hipGetLastError();
 }
 

@@ -161,6 +161,17 @@ class Engine:
         check(lib().jmm_set_steps(self.h, _dp(ms) if ms is not None else None,
                                   _dp(mv) if mv is not None else None), "jmm_set_steps")
 
+    def scaled_energy(self, L_try):
+        """FP64 energy of every chain rescaled to L_try[c] (state unchanged)."""
+        Lt = np.ascontiguousarray(L_try, dtype=np.float64)
+        U = np.zeros(self.n)
+        check(lib().jmm_scaled_energy(self.h, _dp(Lt), _dp(U)), "jmm_scaled_energy")
+        return U
+
+    def set_lengths(self, L_new):
+        Ln = np.ascontiguousarray(L_new, dtype=np.float64)
+        check(lib().jmm_set_lengths(self.h, _dp(Ln)), "jmm_set_lengths")
+
     def perturb_energy(self, chain: int, delta: float):
         check(lib().jmm_perturb_energy(self.h, chain, delta), "jmm_perturb_energy")
 

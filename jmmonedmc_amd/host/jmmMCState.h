@@ -39,7 +39,7 @@ int updateThermo(struct MCState *mcs);          // :1941
 int ECheck(struct MCState *mcs);                // :1965
 int maxDisAdjust(struct MCState *mcs);          // :2100 Swendsen step adaptation
 int maxDVAdjust(struct MCState *mcs);           // :2120 Swendsen box-step adaptation
-int relaxVolume(struct MCState *mcs);           // :2396 (not part of this engine)
+int relaxVolume(struct MCState *mcs);           // :2396 Newton relaxation of the box length
 unsigned long getStepNum(struct MCState *mcs);  // :2777
 int getRelaxFlag(struct MCState *mcs);
 bool getRestartFlag(struct MCState *mcs);
@@ -51,5 +51,6 @@ int printAcc(struct MCState *mcs);              // :2151
 struct jmm_engine *getEngine(struct MCState *mcs);
 double getEnergy(struct MCState *mcs, int chain);
 double getLength(struct MCState *mcs, int chain);
+int isRelaxStep(struct MCState *mcs);  // src/Main.cpp:173-176 as a predicate
 
 #endif

@@ -74,7 +74,13 @@ int main(int argc, char *argv[])
                 if (isMaxDVAdjust(mcs)) maxDVAdjust(mcs);
             }
             if (isGPrint(mcs)) printG(mcs);
+            if (isRelaxStep(mcs)) relaxVolume(mcs);
             std::fflush(stdout);
+            // every thread has read the step counter for the predicates above
+            // before incrementStep's single region advances it; without this
+            // a late thread could see the next step, call a collective the
+            // others skip, and the team would deadlock
+#pragma omp barrier
         }
     }
     if (failed) {

@@ -567,12 +567,78 @@ int maxDVAdjust(struct MCState *m)
     return 0;
 }
 
-int relaxVolume(struct MCState *)
+// src/jmmMCState.cpp:2396: Newton iteration on the box length.  With
+// h = 0.1, E(l +- h) from the configuration scaled to l +- h, the step is
+// dl = -(P - N T / l + dE/dl) / (d2E/dl2), clamped to 0.1 N and to the
+// bracket of earlier lengths; converged when |dl| < 0.0025 N, at most 20
+// steps.  Every chain relaxes at once; messages report chain 0.
+int relaxVolume(struct MCState *m)
 {
-    return collective<int>([] {
-        std::printf("NOTE: RELAX requested; volume relaxation is not part of this engine, skipped.\n");
-        return 0;
+    return collective<int>([m]() -> int {
+        if (int rc = flush(m)) return rc;
+        const size_t n = m->specs.size();
+        std::vector<jmm_counters> c(n);
+        if (int rc = jmm_get_counters(m->eng, -1, c.data())) return rc;
+        std::vector<double> l(n), E(n), lo(n, 0.0), hi(n, 1e10), Lu(n), Ld(n), Eu(n), Ed(n), Ln(n);
+        std::vector<char> done(n, 0);
+        for (size_t k = 0; k < n; ++k) {
+            l[k] = c[k].L;
+            E[k] = c[k].U;
+        }
+        std::printf("Relaxing the Volume...Starting Length,Energy: %.5G,%.5G\n", l[0], E[0]);
+        const double h = 0.1, N = (double)m->N;
+        for (int count = 0; count < 20; ++count) {
+            std::printf("Relaxation step %d...", count);
+            for (size_t k = 0; k < n; ++k) {
+                Lu[k] = l[k] + h;
+                Ld[k] = l[k] - h;
+            }
+            if (int rc = jmm_scaled_energy(m->eng, Lu.data(), Eu.data())) return rc;
+            if (int rc = jmm_scaled_energy(m->eng, Ld.data(), Ed.data())) return rc;
+            bool all = true;
+            for (size_t k = 0; k < n; ++k) {
+                Ln[k] = l[k];
+                if (done[k]) continue;
+                const double first = (Eu[k] - Ed[k]) / (2 * h);
+                const double second = (Eu[k] - 2.0 * E[k] + Ed[k]) / (h * h);
+                double dl = -(m->inp.P - (N / l[k]) * m->inp.T + first) / second;
+                const double relaxMax = 0.10 * N;
+                if (std::fabs(dl) > relaxMax) dl = dl < 0 ? -relaxMax : relaxMax;
+                if (l[k] + dl > hi[k]) dl = 0.5 * (hi[k] - l[k]);
+                else if (l[k] + dl < lo[k]) dl = 0.5 * (lo[k] - l[k]);
+                if (dl > 0.0) lo[k] = l[k];
+                else hi[k] = l[k];
+                if (std::fabs(dl) < 0.0025 * N) done[k] = 1;
+                else all = false;
+                Ln[k] = l[k] + dl;
+            }
+            if (int rc = jmm_set_lengths(m->eng, Ln.data())) {
+                std::printf("\nWARNING: relaxation stopped: %s\n", jmm_last_error());
+                return rc;
+            }
+            if (int rc = jmm_get_counters(m->eng, -1, c.data())) return rc;
+            for (size_t k = 0; k < n; ++k) {
+                l[k] = c[k].L;
+                E[k] = c[k].U;
+            }
+            m->E = E[0];
+            m->l = l[0];
+            if (all) {
+                std::printf("Relaxation converged. Length,energy: %.10G,%.10G\n", l[0], E[0]);
+                return 0;
+            }
+            std::printf("length,energy: %.5g,%.5G\n", l[0], E[0]);
+        }
+        std::printf("WARNING: Relaxation failed to converge after 20 steps. Length,energy: %.5G,%.5G\n",
+                    l[0], E[0]);
+        return 1;
     });
+}
+
+int isRelaxStep(struct MCState *m)
+{
+    // src/Main.cpp:173-176: every 10000 trials below step 1e6 when RELAX is set
+    return m->inp.relaxFlag > 0 && m->sn < 1000000 && crossed(m, 10000);
 }
 
 unsigned long getStepNum(struct MCState *m) { return m->sn; }

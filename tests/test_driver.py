@@ -250,3 +250,28 @@ def test_driver_step_adaptation(gpu, driver, tmp_path):
             want = min(hi, max(lo, prev * math.log(a * 0.5 + b) / math.log(a * ratio + b)))
             assert float(new) == pytest.approx(want, rel=2e-4), (key, prev, ratio, new)
             prev = float(new)
+
+
+@pytest.mark.gpu
+def test_driver_relax_volume(gpu, driver, tmp_path):
+    """RELAX (relaxVolume, src/jmmMCState.cpp:2396) on the harmonic
+    nearest-neighbour chain between walls, where the relaxed length of the
+    scaled lattice solves P - N T / l + 2 (N-1)(l/N - 1)/N = 0 (E = (N-1)
+    (l/N - 1)^2 for lattice gaps l/N): the Newton iteration reaches that root
+    and reports convergence."""
+    from scipy.optimize import brentq
+    inp = tmp_path / "INPUT"
+    N, P, T = 50, 1.0, 0.3
+    inp.write_text(f"ENSEMBLE NPT\nN {N}\nP {P}\nT {T}\nPOT HARMONIC\nNBN 1\nMAXSTEP 0.1\nMAXDV 0.5\n"
+                   f"NUMSTEPS {N + 1}\nCPI {N + 1}\nTPI {N + 1}\nSEED 2\nRELAX\nBOUNDARY WALLS\n"
+                   "LSTART 75\nFORM BATCHED_FP64\n")
+    print("[gpu] driver relax", flush=True)
+    r = subprocess.run([driver, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Relaxing the Volume...Starting Length,Energy: 75" in r.stdout
+    m = re.search(r"Relaxation converged\. Length,energy: (\S+),(\S+)", r.stdout)
+    assert m, r.stdout[-3000:]
+    lf, Ef = float(m.group(1)), float(m.group(2))
+    root = brentq(lambda l: P - N * T / l + 2 * (N - 1) * (l / N - 1) / N, 20.0, 75.0)
+    assert abs(lf - root) < 0.01, (lf, root)
+    assert Ef == pytest.approx((N - 1) * (lf / N - 1) ** 2, rel=1e-8)

@@ -194,3 +194,38 @@ def test_step_sizes_changed_between_sweeps_replay_oracle(gpu, oracle, form):
             assert np.array_equal(logs[c].ravel(), olog), c
             assert np.array_equal(x, ox) and L == oL
             assert rel(U, oU) <= 1e-10
+
+
+@pytest.mark.parametrize("case", ["c1_walls_truncated", "npt_periodic_log", "mixed_batch"])
+def test_scaled_energy_and_set_lengths(gpu, oracle, case):
+    """jmm_scaled_energy (calculateEnergyOfTrialVolumeChange) and
+    jmm_set_lengths (moveVolume): energies of configurations scaled by 0.97
+    and 1.04 equal the oracle's all-pairs energy of x * s at length L s;
+    after set_lengths the positions are x * (L'/L) bit for bit."""
+    print(f"[gpu] scaled energy {case}", flush=True)
+    chains = CASES[case]()
+    with Engine(chains, form="batched_fp64", sample_capacity=32, block_sweeps=8) as e:
+        e.run(12)
+        xs, Ls, Us = e.states()
+        for s in (0.97, 1.04):
+            U = e.scaled_energy(Ls * s)
+            for c, ch in enumerate(chains):
+                spec = oracle_spec(oracle, ch)
+                sc = (Ls[c] * s) / Ls[c]
+                want = oracle.brute_energy_x(spec, xs[c] * sc, Ls[c] * s)
+                if np.isinf(want):                      # a hard core overlaps
+                    assert U[c] == want, (c, s, U[c])
+                    continue
+                assert rel(U[c], want) <= 1e-12 or abs(U[c] - want) <= 1e-12, (c, s, U[c], want)
+        _, Ls2, Us2 = e.states()
+        assert np.array_equal(Ls2, Ls) and np.array_equal(Us2, Us)     # state untouched
+        Lnew = Ls * 1.02
+        e.set_lengths(Lnew)
+        xs3, Ls3, Us3 = e.states()
+        for c, ch in enumerate(chains):
+            assert np.array_equal(xs3[c], xs[c] * (Lnew[c] / Ls[c]))
+            assert Ls3[c] == Lnew[c]
+            spec = oracle_spec(oracle, ch)
+            assert rel(Us3[c], oracle.brute_energy_x(spec, xs3[c], Lnew[c])) <= 1e-12
+        with pytest.raises(JmmError, match="hard core|bad length|walls"):
+            e.set_lengths(Lnew * 0.2)
