@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-sweeps", type=int, default=0, help="CPU baseline sweeps (0 = auto ~15 s)")
     p.add_argument("--graph", type=int, default=1)
+    p.add_argument("--tune", type=int, default=1,
+                   help="choose the batched forms' launch parameters with the autotuner first")
     return p.parse_args()
 
 
@@ -76,10 +78,12 @@ def block_sem(x, nblocks=20):
     return float(m.std() / math.sqrt(nblocks))
 
 
-def run_form(chains, form, steps, warmup, bs, graph, label):
+def run_form(chains, form, steps, warmup, bs, graph, label, opts=None):
     from jmmonedmc_amd.engine import Engine
     cap = max(steps, warmup, 1) * bs
-    eng = Engine(chains, form=form, sample_capacity=cap, block_sweeps=bs, use_graph=bool(graph))
+    o = dict(use_graph=bool(graph))
+    o.update(opts or {})
+    eng = Engine(chains, form=form, sample_capacity=cap, block_sweeps=bs, **o)
     for _ in range(warmup):
         eng.run(bs)
     eng.sync()
@@ -142,11 +146,19 @@ def main():
     chains, desc = workload(a.workload)
     n_trials_per_sweep = sum(ch.N for ch in chains)
 
-    fp32 = run_form(chains, "batched_fp32", a.steps, a.warmup, a.block_sweeps, a.graph, "batched_fp32")
+    tuned = None
+    opts = {}
+    if a.tune:
+        from jmmonedmc_amd.tune import tune
+        print("[bench] autotuning launch parameters", file=sys.stderr, flush=True)
+        tuned = tune(chains, form="batched_fp32", block_sweeps=a.block_sweeps)
+        opts = dict(tuned.best)
+    fp32 = run_form(chains, "batched_fp32", a.steps, a.warmup, a.block_sweeps, a.graph, "batched_fp32",
+                    opts)
     res = {}
     if "fp64" in forms:
         res["batched_fp64"] = run_form(chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps,
-                                       a.graph, "batched_fp64")
+                                       a.graph, "batched_fp64", opts)
     if "plain" in forms and a.plain_steps > 0:
         res["plain_fp64"] = run_form(chains, "plain_fp64", a.plain_steps, 0, a.block_sweeps, 0,
                                      "plain_fp64")
@@ -161,7 +173,7 @@ def main():
         import dataclasses
         ctl_chains = [dataclasses.replace(ch, seed=ch.seed + 1_000_003) for ch in chains]
         ctl = run_form(ctl_chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps, a.graph,
-                       "batched_fp64 control (independent seeds, untimed for the value)")
+                       "batched_fp64 control (independent seeds, untimed for the value)", opts)
         U_ctl = ctl["samples"]["U"]
         h = U_ctl.shape[1] // 2
 
@@ -171,10 +183,22 @@ def main():
                 se = math.hypot(block_sem(Ua[c, h:]), block_sem(U_ctl[c, h:]))
                 if se > 0:
                     z[c] = (Ua[c, h:].mean() - U_ctl[c, h:].mean()) / se
-            return z[np.isfinite(z)]
+            return z
 
-        z32 = zscores(fp32["samples"]["U"])
-        z64 = zscores(res["batched_fp64"]["samples"]["U"])
+        z32a = zscores(fp32["samples"]["U"])
+        z64a = zscores(res["batched_fp64"]["samples"]["U"])
+        z32, z64 = z32a[np.isfinite(z32a)], z64a[np.isfinite(z64a)]
+        # chains whose control run is stationary over the compared half
+        # (third and fourth quarter means within 3 combined SE): slowly
+        # equilibrating chains drift there and their block SE understates
+        # the error in both comparisons
+        q = U_ctl.shape[1] // 4
+        stat = np.zeros(len(chains), dtype=bool)
+        for c in range(len(chains)):
+            a3, a4 = U_ctl[c, 2 * q:3 * q], U_ctl[c, 3 * q:]
+            se = math.hypot(block_sem(a3, 10), block_sem(a4, 10))
+            stat[c] = se > 0 and abs(a3.mean() - a4.mean()) <= 3 * se
+        z32s, z64s = z32a[stat & np.isfinite(z32a)], z64a[stat & np.isfinite(z64a)]
         accuracy = {"observable": "<U> per chain over the second half of the run, 20-block SEM, "
                                   "against an FP64 run with independent seeds",
                     "chains_compared": int(z32.size),
@@ -183,7 +207,10 @@ def main():
                     "mean_z": float(z32.mean()), "null_mean_z": float(z64.mean()),
                     "std_z": float(z32.std()), "null_std_z": float(z64.std()),
                     "max_abs_z": float(np.abs(z32).max()),
-                    "null_max_abs_z": float(np.abs(z64).max())}
+                    "null_max_abs_z": float(np.abs(z64).max()),
+                    "stationary_chains": int(z32s.size),
+                    "stationary_frac_within_3sigma": float((np.abs(z32s) <= 3).mean()) if z32s.size else None,
+                    "stationary_null_frac_within_3sigma": float((np.abs(z64s) <= 3).mean()) if z64s.size else None}
 
     # roofline of the dominant kernel (the batched FP32 sweep kernel)
     achieved = fp32["flops"] / (fp32["kernel_ms"] * 1e-3) / 1e12
@@ -223,7 +250,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "n_chains": len(chains), "particles": n_trials_per_sweep,
                    "sweeps_timed": a.steps * a.block_sweeps, "block_sweeps": a.block_sweeps,
-                   "form": "batched_fp32", "parallelism": "1 GPU, chains packed into workgroups"},
+                   "form": "batched_fp32", "parallelism": "1 GPU, chains packed into workgroups",
+                   "launch": opts or {"use_graph": bool(a.graph)}},
+        "autotune_ms_per_sweep": tuned and tuned.ms_per_sweep,
         "chain_sweeps_per_s": len(chains) * a.steps * a.block_sweeps / fp32["wall_s"],
         "sample_readback": {"seconds": fp32["readback_s"],
                             "bytes": int(fp32["samples"].size * 24),

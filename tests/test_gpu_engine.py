@@ -229,3 +229,29 @@ def test_scaled_energy_and_set_lengths(gpu, oracle, case):
             assert rel(Us3[c], oracle.brute_energy_x(spec, xs3[c], Lnew[c])) <= 1e-12
         with pytest.raises(JmmError, match="hard core|bad length|walls"):
             e.set_lengths(Lnew * 0.2)
+
+
+def test_autotuner_candidates_give_identical_chains(gpu, tmp_path):
+    """The tuned parameters only change packing and launch: every candidate
+    produces bit-identical samples and states (FP32 form, mixed batch), and
+    the tuner returns one of them, cached by the chain-mix signature."""
+    from jmmonedmc_amd.tune import DEFAULT_CANDIDATES, tune
+    print("[gpu] autotuner invariance", flush=True)
+    chains = CASES["mixed_batch"]() + [Chain(N=2000, L0=4000.0, seed=9, chain_id=77, **lj())]
+    ref = None
+    for cand in DEFAULT_CANDIDATES:
+        with Engine(chains, form="batched_fp32", sample_capacity=40, block_sweeps=20, **cand) as e:
+            e.run(40)
+            smp = e.read_samples(40)
+            xs, Ls, Us = e.states()
+            got = (smp, np.concatenate(xs), Ls, Us)
+        if ref is None:
+            ref = got
+            continue
+        assert np.array_equal(got[0], ref[0]), cand
+        assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), cand
+    cache = str(tmp_path / "tune.json")
+    r1 = tune(chains, block_sweeps=20, blocks=1, cache=cache)
+    assert r1.best in DEFAULT_CANDIDATES and len(r1.ms_per_sweep) == len(DEFAULT_CANDIDATES)
+    r2 = tune(chains, block_sweeps=20, blocks=1, cache=cache)
+    assert r2.best == r1.best and r2.ms_per_sweep == r1.ms_per_sweep
