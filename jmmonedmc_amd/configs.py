@@ -61,9 +61,13 @@ def c4(n_chains=1024, N=1024, eps=1.0, sigma=1.0):
             for c in range(n_chains)]
 
 
-def c5(n_chains=64, N=65536):
+def c5(n_chains=64, N=65536, bcap=1024):
+    """64 Yukawa chains of 65536 particles.  The window (41 partners per side)
+    needs blocks of at least 43 particles; bcap = 1024 gives 1024 blocks of 64
+    (64 phases per sweep) where the default 256 would give 256 phases."""
     return [Chain(N=N, rho=0.8, T=1.5, P=1.0, ensemble=nat.NPT, vol_move=nat.VOL_LOG, max_dv=0.01,
-                  max_step=0.5, seed=c, chain_id=c, eci=100, **YUKAWA) for c in range(n_chains)]
+                  max_step=0.5, seed=c, chain_id=c, eci=100, bcap=bcap, **YUKAWA)
+            for c in range(n_chains)]
 
 
 def flop_per_pair(potential: int) -> float:
