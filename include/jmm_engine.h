@@ -88,7 +88,11 @@ typedef struct jmm_engine_opts {
     uint32_t use_graph;       /* 1: replay sweep blocks from a captured hipGraph */
     uint32_t log_accepts;     /* 1: record one byte per displacement trial (tests) */
     uint32_t lds_budget;      /* batched forms: LDS bytes per workgroup (0 = 64 KiB) */
-    uint32_t reserved[4];
+    uint32_t seg_lds;         /* batched forms: a chain whose LDS image exceeds this many bytes
+                                 (0 = the device's LDS per workgroup) is split into segments of
+                                 consecutive blocks, one workgroup each, that synchronise per
+                                 phase; a chain that cannot be split runs from HBM */
+    uint32_t reserved[3];
 } jmm_engine_opts;
 
 /* One per chain per sweep (the per-sweep form of the reference's per-trial
@@ -125,6 +129,21 @@ typedef struct jmm_hist_spec {
     uint32_t reserved;
     double rbw, gsw, gbw;
 } jmm_hist_spec;
+
+/* Where a chain runs (see jmm_chain_layout). */
+enum {
+    JMM_MODE_LDS = 0,    /* whole chain in one workgroup's LDS */
+    JMM_MODE_HBM = 1,    /* positions read and written in HBM */
+    JMM_MODE_SPLIT = 2   /* split over several cooperating workgroups */
+};
+
+typedef struct jmm_layout {
+    uint32_t B;          /* particle blocks (one lane each) */
+    uint32_t w;          /* partner window: neighbours a trial may reach per side */
+    uint32_t mode;       /* JMM_MODE_* */
+    uint32_t segments;   /* workgroups the chain spans (1 unless split) */
+    uint64_t lds_bytes;  /* LDS image of the chain, or of one segment when split */
+} jmm_layout;
 
 typedef struct jmm_engine jmm_engine;
 
@@ -242,6 +261,12 @@ int jmm_hist_sample(jmm_engine *e);
  * rhoA/ns/rbw and gA/ns/gsw/gbw).  Either array may be NULL.  reset = 1
  * clears the counts.  Synchronous. */
 int jmm_hist_read(jmm_engine *e, uint64_t *rho, uint64_t *g, uint64_t *n_samples, int reset);
+
+/* Placement of chain `chain` (external index): block count, window, memory
+ * mode and the number of workgroups it spans.  A diagnostic with no
+ * reference counterpart (the reference's chains are host loops).
+ * Synchronous only in that it reads host-side tables. */
+int jmm_chain_layout(jmm_engine *e, int chain, jmm_layout *out);
 
 /* Kernel timing of the sweep kernel, from HIP events recorded on the
  * engine's stream around every sweep-block launch since the last reset:

@@ -24,7 +24,7 @@ EXPORTED = [
     "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
     "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read", "jmm_set_steps",
-    "jmm_scaled_energy", "jmm_set_lengths",
+    "jmm_scaled_energy", "jmm_set_lengths", "jmm_chain_layout",
 ]
 
 
@@ -43,7 +43,8 @@ class EngineOpts(C.Structure):
     _fields_ = [
         ("form", C.c_int32), ("device", C.c_int32), ("sample_capacity", C.c_uint32),
         ("block_sweeps", C.c_uint32), ("wg_threads", C.c_uint32), ("use_graph", C.c_uint32),
-        ("log_accepts", C.c_uint32), ("lds_budget", C.c_uint32), ("reserved", C.c_uint32 * 4),
+        ("log_accepts", C.c_uint32), ("lds_budget", C.c_uint32), ("seg_lds", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -55,6 +56,15 @@ class HistSpec(C.Structure):
     _fields_ = [("rhonb", C.c_uint32), ("gns", C.c_uint32), ("gnb", C.c_uint32),
                 ("reserved", C.c_uint32), ("rbw", C.c_double), ("gsw", C.c_double),
                 ("gbw", C.c_double)]
+
+
+MODE_LDS, MODE_HBM, MODE_SPLIT = 0, 1, 2
+MODES = {MODE_LDS: "lds", MODE_HBM: "hbm", MODE_SPLIT: "split"}
+
+
+class Layout(C.Structure):
+    _fields_ = [("B", C.c_uint32), ("w", C.c_uint32), ("mode", C.c_uint32),
+                ("segments", C.c_uint32), ("lds_bytes", C.c_uint64)]
 
 
 class Counters(C.Structure):

@@ -242,6 +242,14 @@ struct jmm_engine {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     uint64_t ev_sweeps = 0;
     uint64_t sweeps_run = 0;            // sweeps launched since creation
+    // chains split over several workgroups (segments)
+    std::vector<jmm_layout> layout;  // external index
+    uint32_t n_segs = 0, seg_wgmax = 1, seg_ctrl = 0, n_ctrl = 0;
+    size_t seg_lds_bytes = 0;
+    DevSeg *d_segs = nullptr;
+    double *d_pub_head = nullptr, *d_pub_tail = nullptr, *d_part = nullptr;
+    unsigned long long *d_ctr = nullptr;
+    uint32_t *d_seg_err = nullptr;
     std::vector<double> step_cap;       // max_step at creation, per internal chain
     // rho(x)/g(x) sampling (jmm_hist_configure)
     uint32_t hist_every = 0;
@@ -281,6 +289,41 @@ int launch_block(jmm_engine *e, uint32_t m)
                                   e->stream));
         }
     }
+    if (e->n_segs) {
+        HIPCHK(launch_begin(&e->d_ctrl[e->seg_ctrl], m, e->stream));
+        SegArgs g{};
+        g.a.chains = e->d_chains;
+        g.a.states = e->d_states;
+        g.a.x = e->d_x;
+        g.a.ctrl = &e->d_ctrl[e->seg_ctrl];
+        g.a.sU = e->d_sU;
+        g.a.sL = e->d_sL;
+        g.a.sA = e->d_sA;
+        g.a.acclog = e->d_log;
+        g.a.n_sweeps = m;
+        g.a.n_chains = (uint32_t)e->n;
+        g.a.cap = e->cap;
+        g.segs = e->d_segs;
+        g.pub_head = e->d_pub_head;
+        g.pub_tail = e->d_pub_tail;
+        g.ctr = e->d_ctr;
+        g.part = e->d_part;
+        g.err = e->d_seg_err;
+        g.wgmax = e->seg_wgmax;
+        g.n_segs = e->n_segs;
+        HIPCHK(launch_segmented(g, e->form == JMM_FORM_BATCHED_FP32, e->seg_lds_bytes, e->stream));
+    }
+    return 0;
+}
+
+// Split chains of the segmented path wait for each other with bounded
+// polls; a wait that ran out sets this flag.
+int check_segments(jmm_engine *e)
+{
+    if (!e->n_segs) return 0;
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, e->d_seg_err, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return fail(JMM_EHIP, "segmented chains: a synchronisation wait timed out; results are invalid");
     return 0;
 }
 
@@ -304,11 +347,12 @@ struct PackOut {
 };
 
 PackOut pack(const std::vector<jmm_chain_spec> &specs, const std::vector<DevChain> &ch, int T,
-             size_t lds_doubles, size_t lds_max_doubles)
+             size_t lds_doubles, size_t lds_max_doubles, const std::vector<uint32_t> &segS)
 {
     const int n = (int)specs.size();
-    std::vector<uint32_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0u);
+    std::vector<uint32_t> idx;
+    for (int c = 0; c < n; ++c)
+        if (!segS[c]) idx.push_back((uint32_t)c);
     // chains of one bundle share everything but N, T, P, L0, seed and stream
     // (the kernel reads the shared part once per bundle)
     auto key = [&](uint32_t c) {
@@ -417,6 +461,12 @@ int jmm_destroy(jmm_engine *e)
     (void)hipFree(e->d_all);
     (void)hipFree(e->d_rho);
     (void)hipFree(e->d_g);
+    (void)hipFree(e->d_segs);
+    (void)hipFree(e->d_pub_head);
+    (void)hipFree(e->d_pub_tail);
+    (void)hipFree(e->d_part);
+    (void)hipFree(e->d_ctr);
+    (void)hipFree(e->d_seg_err);
     for (auto &G : e->grp) {
         (void)hipFree(G.d_bundles);
         (void)hipFree(G.d_list);
@@ -469,7 +519,31 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     int n_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device));
 
-    // internal chain order and bundles
+    // chains whose LDS image does not fit one workgroup: split into S
+    // segments of B/S blocks (S the smallest power of two whose segment image
+    // fits and has at most 256 lanes); a chain that cannot be split stays
+    // whole and runs from HBM
+    std::vector<uint32_t> segS(n_chains, 0);
+    if (e->form != JMM_FORM_PLAIN_FP64) {
+        const size_t dev_max = (size_t)lds_max / 8 - 64;
+        const size_t thr = o.seg_lds ? std::min<size_t>(o.seg_lds / 8, dev_max) : dev_max;
+        for (int c = 0; c < n_chains; ++c) {
+            if ((size_t)ch[c].lds_len <= thr) continue;
+            for (uint32_t S = 2; S <= ch[c].B; S *= 2) {
+                const uint32_t Bs = ch[c].B / S;
+                const size_t img = 2 * (size_t)ch[c].wg + (size_t)Bs * ch[c].Sp;
+                if (Bs <= 256 && img <= thr) {
+                    segS[c] = S;
+                    break;
+                }
+            }
+        }
+    }
+    maxB = 1;
+    for (int c = 0; c < n_chains; ++c)
+        if (!segS[c]) maxB = std::max(maxB, ch[c].B);
+
+    // internal chain order and bundles (segmented chains last)
     std::vector<DevBundle> bundles[2];
     std::vector<DevSlot> slots;
     std::vector<uint32_t> order(n_chains);
@@ -480,10 +554,34 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         if (maxB > 1024) return bail(fail(JMM_EINVAL, "jmm_create: more than 1024 blocks in a chain"));
         size_t budget = o.lds_budget ? o.lds_budget : 65536;
         budget = std::min<size_t>(budget, (size_t)lds_max);
-        PackOut p = pack(e->specs, ch, e->T, budget / 8, (size_t)lds_max / 8 - 64);
+        PackOut p = pack(e->specs, ch, e->T, budget / 8, (size_t)lds_max / 8 - 64, segS);
         order = p.order;
+        for (int c = 0; c < n_chains; ++c)
+            if (segS[c]) order.push_back((uint32_t)c);
         slots = p.slots;
         for (int g = 0; g < 2; ++g) bundles[g] = p.b[g];
+    }
+    e->layout.assign(n_chains, jmm_layout{});
+    for (int c = 0; c < n_chains; ++c) {
+        jmm_layout &l = e->layout[c];
+        l.B = ch[c].B;
+        l.w = ch[c].w;
+        l.segments = 1;
+        if (e->form == JMM_FORM_PLAIN_FP64) {
+            l.B = 1;
+            l.lds_bytes = sizeof(double) * specs[c].N;
+            l.mode = l.lds_bytes > (uint64_t)lds_max ? JMM_MODE_HBM : JMM_MODE_LDS;
+            if (l.mode == JMM_MODE_HBM) l.lds_bytes = 0;
+        } else if (segS[c]) {
+            l.mode = JMM_MODE_SPLIT;
+            l.segments = segS[c];
+            l.lds_bytes = sizeof(double) * (2 * (uint64_t)ch[c].wg + (uint64_t)(ch[c].B / segS[c]) * ch[c].Sp);
+        } else if ((size_t)ch[c].lds_len > (size_t)lds_max / 8 - 64) {
+            l.mode = JMM_MODE_HBM;
+        } else {
+            l.mode = JMM_MODE_LDS;
+            l.lds_bytes = sizeof(double) * (uint64_t)ch[c].lds_len;
+        }
     }
     e->ext_of_int = order;
     e->int_of_ext.assign(n_chains, 0);
@@ -581,9 +679,59 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             e->grp.push_back(G);
         }
     }
-    if (int rc = dalloc(&e->d_ctrl, std::max<size_t>(1, e->grp.size()))) return bail(rc);
+    // segmented chains: one cooperative launch, one workgroup per segment
     {
-        std::vector<Ctrl> zc(std::max<size_t>(1, e->grp.size()));
+        std::vector<DevSeg> segv;
+        uint32_t wgmax = 1;
+        size_t img = 0;
+        for (int k = 0; k < n_chains; ++k) {
+            const uint32_t S = segS[order[k]];
+            if (!S) continue;
+            const DevChain &d = e->hch[k];
+            const uint32_t first = (uint32_t)segv.size();
+            for (uint32_t si = 0; si < S; ++si) {
+                DevSeg g{};
+                g.chain = (uint32_t)k;
+                g.s = si;
+                g.S = S;
+                g.Bs = d.B / S;
+                g.first = first;
+                segv.push_back(g);
+            }
+            wgmax = std::max(wgmax, d.wg);
+            img = std::max(img, 2 * (size_t)d.wg + (size_t)(d.B / S) * d.Sp);
+        }
+        if (!segv.empty()) {
+            const int fp32 = e->form == JMM_FORM_BATCHED_FP32;
+            e->n_segs = (uint32_t)segv.size();
+            e->seg_wgmax = wgmax;
+            e->seg_lds_bytes = img * sizeof(double);
+            if (int rc = set_lds_attr(segmented_kernel(fp32), e->seg_lds_bytes)) return bail(rc);
+            int occ = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, segmented_kernel(fp32), 256,
+                                                                e->seg_lds_bytes));
+            if ((uint64_t)occ * (uint64_t)n_cu < e->n_segs)
+                return bail(fail(JMM_EINVAL, "jmm_create: split chains need %u workgroups resident at once, "
+                                             "the device holds %d", e->n_segs, occ * n_cu));
+            if (int rc = dalloc(&e->d_segs, segv.size())) return bail(rc);
+            HIPCHK(hipMemcpy(e->d_segs, segv.data(), sizeof(DevSeg) * segv.size(), hipMemcpyHostToDevice));
+            if (int rc = dalloc(&e->d_pub_head, (size_t)e->n_segs * wgmax)) return bail(rc);
+            if (int rc = dalloc(&e->d_pub_tail, (size_t)e->n_segs * wgmax)) return bail(rc);
+            if (int rc = dalloc(&e->d_part, (size_t)e->n_segs * 2 * SEG_FIELDS * SEG_WAVES)) return bail(rc);
+            if (int rc = dalloc(&e->d_ctr, e->n_segs)) return bail(rc);
+            if (int rc = dalloc(&e->d_seg_err, 1)) return bail(rc);
+            std::vector<unsigned long long> z(e->n_segs, 0ull);
+            HIPCHK(hipMemcpy(e->d_ctr, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+            const uint32_t zero = 0;
+            HIPCHK(hipMemcpy(e->d_seg_err, &zero, sizeof zero, hipMemcpyHostToDevice));
+            e->seg_ctrl = (uint32_t)e->grp.size();
+            e->use_graph = 0;  // cooperative launches are not captured
+        }
+    }
+    e->n_ctrl = (uint32_t)e->grp.size() + (e->n_segs ? 1u : 0u);
+    if (int rc = dalloc(&e->d_ctrl, std::max<size_t>(1, e->n_ctrl))) return bail(rc);
+    {
+        std::vector<Ctrl> zc(std::max<size_t>(1, e->n_ctrl));
         HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
     }
 
@@ -727,7 +875,7 @@ int jmm_synchronize(jmm_engine *e)
     if (!e) return fail(JMM_EINVAL, "jmm_synchronize: null engine");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
-    return 0;
+    return check_segments(e);
 }
 
 int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *sweeps, int reset)
@@ -760,6 +908,7 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
     if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_samples: null argument");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
+    if (int rc = check_segments(e)) return rc;
     const uint32_t n = e->write_pos;
     if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_samples: %u sweeps pending, room for %u", n, max_sweeps);
     if (n && !out) return fail(JMM_EINVAL, "jmm_read_samples: out is NULL");
@@ -781,7 +930,7 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
             o.acc_v = A[r].y;
         }
     e->write_pos = 0;
-    std::vector<Ctrl> zc(std::max<size_t>(1, e->grp.size()));
+    std::vector<Ctrl> zc(std::max<size_t>(1, e->n_ctrl));
     HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
     *n_out = n;
     return 0;
@@ -793,6 +942,7 @@ int jmm_read_accept_log(jmm_engine *e, uint8_t *out, uint32_t max_sweeps, uint32
     if (!e->d_log) return fail(JMM_ESTATE, "engine was created without log_accepts");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
+    if (int rc = check_segments(e)) return rc;
     const uint32_t n = e->write_pos;
     if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_accept_log: %u sweeps pending, room for %u", n, max_sweeps);
     for (int k = 0; k < e->n && n; ++k) {
@@ -810,6 +960,7 @@ int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uin
     if (!e || chain < 0 || chain >= e->n) return fail(JMM_EINVAL, "jmm_get_state: bad chain");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
+    if (int rc = check_segments(e)) return rc;
     const uint32_t k = e->int_of_ext[chain];
     if (x) HIPCHK(hipMemcpy(x, e->d_x + e->hch[k].x_off, sizeof(double) * e->hch[k].N, hipMemcpyDeviceToHost));
     DevState st;
@@ -820,11 +971,19 @@ int jmm_get_state(jmm_engine *e, int chain, double *x, double *L, double *U, uin
     return 0;
 }
 
+int jmm_chain_layout(jmm_engine *e, int chain, jmm_layout *out)
+{
+    if (!e || !out || chain < 0 || chain >= e->n) return fail(JMM_EINVAL, "jmm_chain_layout: bad argument");
+    *out = e->layout[chain];
+    return 0;
+}
+
 int jmm_get_states(jmm_engine *e, double *x, double *L, double *U)
 {
     if (!e) return fail(JMM_EINVAL, "jmm_get_states: null engine");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
+    if (int rc = check_segments(e)) return rc;
     if (x) {
         std::vector<double> hx(e->total_N);
         HIPCHK(hipMemcpy(hx.data(), e->d_x, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));

@@ -91,6 +91,31 @@ struct ObsArgs {
     uint32_t standalone;
 };
 
+// A chain too large for one workgroup's LDS is split into S segments of Bs
+// consecutive blocks, one workgroup each (cooperative launch: all resident).
+struct DevSeg {
+    uint32_t chain;  // internal chain index
+    uint32_t s, S;   // segment index, segments of the chain
+    uint32_t Bs;     // blocks (lanes) in each segment
+    uint32_t first;  // index of the chain's segment 0 in the per-segment arrays
+    uint32_t pad[3];
+};
+
+// Per-segment partial sums: [segment][sweep parity][field][wave of the
+// segment's workgroup], fields dU, box-length dU, accepts, overlaps,
+// recomputed U, final counters.
+constexpr uint32_t SEG_FIELDS = 8, SEG_WAVES = 4;
+
+struct SegArgs {
+    SweepArgs a;
+    const DevSeg *segs;
+    double *pub_head, *pub_tail;  // [segment][wgmax]: first / last wg positions
+    unsigned long long *ctr;      // [segment]: synchronisation points passed
+    double *part;                 // [segment][2][SEG_FIELDS][SEG_WAVES]
+    uint32_t *err;                // set when a wait times out
+    uint32_t wgmax, n_segs;
+};
+
 // rho(x) and g(x) histograms of every chain's current configuration.
 struct HistArgs {
     const DevChain *chains;
@@ -114,6 +139,8 @@ hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, h
 hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s);
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s);
 hipError_t launch_hist(const HistArgs &a, hipStream_t s);
+const void *segmented_kernel(int fp32);
+hipError_t launch_segmented(const SegArgs &a, int fp32, size_t lds, hipStream_t s);
 hipError_t launch_rescale(const DevChain *chains, const DevState *st, double *x, const double *Lnew,
                           uint32_t n_chains, uint32_t parts, hipStream_t s);
 }  // namespace jmm

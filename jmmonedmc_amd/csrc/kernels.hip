@@ -883,6 +883,437 @@ k_sweep_batched(SweepArgs a)
     }
 }
 
+// ---------------------------------------------------------- segmented chains
+
+// Synchronisation between the workgroups (segments) of one chain.  Every
+// wave of a segment fences its boundary stores at agent scope before the
+// workgroup barrier; thread 0 then advances the segment's counter.  A waiter
+// polls its partners' counters with agent-scope loads and fences before the
+// workgroup reads what they published.  Every wait is bounded: after 2^22
+// polls the error flag is set and all later waits return at once, so the
+// grid drains in bounded time and the host reports the error.
+__device__ __forceinline__ void seg_signal(unsigned long long *ctr, unsigned long long v)
+{
+    __threadfence();
+    __hip_atomic_store(ctr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __noinline__ void seg_wait(const unsigned long long *ctr, unsigned long long target,
+                                      uint32_t *err)
+{
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    for (uint32_t it = 0; it < (1u << 22); ++it) {
+        if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+            __threadfence();
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double seg_load(const double *p)
+{
+    const unsigned long long v =
+        __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+
+// Slot of a segment's partial sum: [segment][sweep parity][field][wave].
+__device__ __forceinline__ uint64_t seg_pidx(uint32_t j, uint32_t par, uint32_t f, uint32_t wv)
+{
+    return (((uint64_t)j * 2 + par) * SEG_FIELDS + f) * SEG_WAVES + wv;
+}
+
+// One segment of a chain: blocks s*Bs .. s*Bs+Bs-1, one lane each, in LDS
+// with halos of wg positions on either side that hold the neighbouring
+// segments' boundary particles (periodic images at the chain's ends, as the
+// single-workgroup layout's ghosts).  The trials, the random numbers, the
+// per-lane sums and the order in which the sums are combined are those of
+// run_bundle_lds, so a split chain produces the same bits as a whole one.
+//
+// Neighbour syncs every K = qb - wg phases (qb the smaller block size): in
+// phase t the last block of a segment reads the right neighbour's first-block
+// particles m <= t + wg - size, updated in phases before the previous sync,
+// and the first block reads the left neighbour's last-block particles
+// p >= size - wg + t, which the neighbour (at most K phases ahead) has not
+// updated yet in this sweep.  So between two syncs every halo value read is
+// the one the sequential schedule reads.  Sync points, the same in every
+// segment of a chain: the initial publication (neighbours); phases
+// K-1, 2K-1, ... and the last phase (neighbours); every sweep's partial sums
+// (all segments); a box-length acceptance (neighbours); a recompute (all);
+// the final counters (all).  Partial sums are double-buffered by sweep
+// parity: a segment is at most one all-segment point ahead of another.
+template <typename Real, int KIND>
+__device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
+                            BatchedScratch<256> &sc, uint32_t soff)
+{
+    const SweepArgs &a = g.a;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ic = sg.chain, S = sg.S, s = sg.s, Bs = sg.Bs;
+    const uint32_t me = sg.first + s;
+    const DevChain &ch = a.chains[ic];
+    const Pot q = ch.pot;
+    const double gmin = ch.gmin, max_step = ch.max_step;
+    const double beta = ch.beta, P = ch.P;
+    const float betaf = ch.betaf;
+    const uint32_t key0 = (uint32_t)ch.seed, key1 = (uint32_t)(ch.seed >> 32);
+    const uint32_t chain_id = ch.chain_id, eci = ch.eci;
+    const uint64_t log_off = ch.log_off;
+    const bool per = (ch.flags & F_PERIODIC) != 0, npt = (ch.flags & F_NPT) != 0;
+    const uint32_t N = ch.N, w = ch.w, wg = ch.wg, Sp = ch.Sp, smax = ch.smax;
+    const uint32_t qb = N / ch.B, rem = N % ch.B;
+    const bool hasLn = per || s > 0, hasRn = per || s + 1 < S;
+    const uint32_t ln = sg.first + (s + S - 1) % S, rn = sg.first + (s + 1) % S;
+    const uint32_t W = g.wgmax;
+    const uint32_t K = qb > wg ? qb - wg : 1u;
+    const uint32_t nwv = Bs >= 64 ? Bs / 64 : 1u;  // wave totals per segment and field
+
+    LdsChain c;
+    c.R = smem;
+    c.N = N;
+    c.B = Bs;
+    c.w = w;
+    c.wg = wg;
+    c.Sp = Sp;
+    c.qb = qb;
+    c.rem = rem;
+    c.per = per;
+
+    const bool act = tid < Bs;
+    const uint32_t r = tid;
+    const uint32_t b = s * Bs + r;
+    const uint32_t size = act ? qb + (b < rem ? 1u : 0u) : 0u;
+    const uint32_t start = act ? b * qb + (b < rem ? b : rem) : 0u;
+    const uint32_t a0 = wg + r * Sp;
+    const uint32_t gapR = Sp - size;
+    const uint32_t gapL = (r == 0 || !act) ? 0u : Sp - (qb + (b - 1 < rem ? 1u : 0u));
+    const uint32_t seg = act ? Bs : 1u;
+    // the segment's last block
+    const uint32_t bl = s * Bs + Bs - 1;
+    const uint32_t size_last = qb + (bl < rem ? 1u : 0u);
+    const uint32_t a0_last = wg + (Bs - 1) * Sp;
+
+    const double *xg = a.x + ch.x_off;
+    for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = xg[start + t];
+    const DevState st = a.states[ic];
+    double L = st.L, U = st.U, mdrift = st.max_drift;
+    uint64_t sweep = st.sweep;
+    unsigned long long sp = __hip_atomic_load(&g.ctr[me], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+
+    auto publish_all = [&]() {
+        for (uint32_t m = tid; m < wg; m += 256) {
+            g.pub_head[(uint64_t)me * W + m] = c.R[wg + m];
+            g.pub_tail[(uint64_t)me * W + m] = c.R[a0_last + size_last - wg + m];
+        }
+    };
+    auto refresh = [&]() {
+        for (uint32_t m = tid; m < wg; m += 256) {
+            if (hasLn) {
+                const double v = seg_load(&g.pub_tail[(uint64_t)ln * W + m]);
+                c.R[m] = (per && s == 0) ? v - L : v;
+            }
+            if (hasRn) {
+                const double v = seg_load(&g.pub_head[(uint64_t)rn * W + m]);
+                c.R[wg + Bs * Sp + m] = (per && s + 1 == S) ? v + L : v;
+            }
+        }
+        __syncthreads();
+    };
+    auto sync_nb = [&]() {
+        ++sp;
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            seg_signal(&g.ctr[me], sp);
+            if (hasLn) seg_wait(&g.ctr[ln], sp, g.err);
+            if (hasRn) seg_wait(&g.ctr[rn], sp, g.err);
+        }
+        __syncthreads();
+    };
+    auto sync_all = [&]() {
+        ++sp;
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            seg_signal(&g.ctr[me], sp);
+            for (uint32_t j = 0; j < S; ++j)
+                if (j != s) seg_wait(&g.ctr[sg.first + j], sp, g.err);
+        }
+        __syncthreads();
+    };
+    // this segment's wave totals of slot m of the last nd-wide seg_allreduce
+    // (thread 0)
+    auto put = [&](uint32_t par, uint32_t f, uint32_t m, uint32_t nd) {
+        for (uint32_t wv = 0; wv < nwv; ++wv) g.part[seg_pidx(me, par, f, wv)] = sc.rd[wv * nd + m];
+    };
+    // Field f summed over the chain's blocks in the order of seg_allreduce
+    // over B = S*Bs lanes: pairwise trees within 64-lane groups, the group
+    // totals then added in sequence from zero (no sequence when B <= 64).
+    // Every segment reads the same values in the same order: the same bits.
+    __shared__ double tot[SEG_FIELDS];
+    auto gather = [&](uint32_t par, uint32_t f) {
+        if (tid < 64) {
+            double acc = 0.0;
+            if (Bs >= 64) {
+                if (tid == 0)
+                    for (uint32_t j = 0; j < S; ++j)
+                        for (uint32_t wv = 0; wv < nwv; ++wv)
+                            acc += seg_load(&g.part[seg_pidx(sg.first + j, par, f, wv)]);
+            } else {
+                const uint32_t Bt = S * Bs;
+                const uint32_t gs = Bt <= 64 ? S : 64u / Bs;  // segment totals per group
+                const uint32_t ng = Bt <= 64 ? 1u : Bt / 64u;
+                for (uint32_t k = 0; k < ng; ++k) {
+                    double v = tid < gs ? seg_load(&g.part[seg_pidx(sg.first + k * gs + tid, par, f, 0)]) : 0.0;
+#pragma unroll
+                    for (uint32_t off = 1; off < 64; off <<= 1) {
+                        const double p = __shfl_xor(v, (int)off);
+                        if (off < gs) v += p;
+                    }
+                    acc = Bt <= 64 ? v : acc + v;
+                }
+            }
+            if (tid == 0) tot[f] = acc;
+        }
+        __syncthreads();
+    };
+    // integer totals (exact in double), any order
+    auto gather_int = [&](uint32_t par, uint32_t f, uint32_t wv) {
+        if (tid == 0) {
+            double v = 0.0;
+            for (uint32_t j = 0; j < S; ++j) v += seg_load(&g.part[seg_pidx(sg.first + j, par, f, wv)]);
+            tot[f] = v;
+        }
+        __syncthreads();
+    };
+
+    publish_all();
+    sync_nb();
+    refresh();
+
+    uint32_t n_tr = 0, n_wall = 0, n_core = 0, n_terms = 0;
+    uint32_t acc_total = 0, n_vt = 0, n_av = 0, n_rc = 0;
+
+#pragma unroll 1
+    for (uint32_t sw = 0; sw < a.n_sweeps; ++sw, ++sweep) {
+        double dUs = 0.0;
+        uint32_t acc = 0;
+        uint32_t w_md = 0, w_u = 0, w_next_md = 0, w_next_u = 0;
+#pragma unroll 1
+        for (uint32_t t = 0; t < smax; ++t) {
+            if (t < size) {
+                const uint32_t i = start + t;
+                const uint32_t my = a0 + t;
+                if ((t & 1u) == 0) {
+                    uint32_t rr[4];
+                    philox4x32_10(i, (uint32_t)sweep, chain_id, 0u, key0, key1, rr);
+                    w_md = rr[0];
+                    w_u = rr[1];
+                    w_next_md = rr[2];
+                    w_next_u = rr[3];
+                } else {
+                    w_md = w_next_md;
+                    w_u = w_next_u;
+                }
+                const double a1 = u32o(w_md);
+                const double u = u32o(w_u);
+                const double md = ((a1 - 0.5) * 2.0) * max_step;
+                const double xi = c.R[my];
+                const double xn = xi + md;
+                ++n_tr;
+                double dE = 0.0;
+                const bool accd = trial_lds<Real, KIND>(c, q, gmin, beta, betaf, i, t, size, my,
+                                                        gapL, gapR, L, xi, xn, md, u, dE, n_terms,
+                                                        n_wall, n_core);
+                const double xs = accd ? xn : xi;
+                c.R[my] = xs;
+                dUs += accd ? dE : 0.0;
+                acc += accd ? 1u : 0u;
+                if (a.acclog) a.acclog[log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
+                // boundary particles the neighbouring segments read
+                if (r == 0 && t < wg) g.pub_head[(uint64_t)me * W + t] = xs;
+                if (r + 1 == Bs && t + wg >= size) g.pub_tail[(uint64_t)me * W + (t + wg - size)] = xs;
+            }
+            if ((t + 1) % K == 0 || t + 1 == smax) {
+                sync_nb();
+                refresh();
+            } else {
+                __syncthreads();
+            }
+        }
+
+        // box-length trial: per-lane sums over the pairs whose left member
+        // is in the lane's block, across segment boundaries through the halo
+        VolProp vp{};
+        uint32_t bad = 0;
+        double vpart = 0.0;
+        if (npt) {
+            vp = vol_propose(ch, L, sweep);
+            if (vp.valid && act) {
+                double part = 0.0;
+                float sf = 0.0f, cc = 0.0f;
+#pragma unroll 1
+                for (uint32_t t = 0; t < size && !bad; ++t) {
+                    const uint32_t i = start + t;
+                    const uint32_t my = a0 + t;
+                    const double xi = c.R[my], xis = xi * vp.sc;
+                    const uint32_t kR = per ? w : min(w, N - 1 - i);
+#pragma unroll 1
+                    for (uint32_t k = 1; k <= kR; ++k) {
+                        const bool cross = t + k >= size;
+                        const double xj = c.R[my + k + (cross ? gapR : 0)];
+                        double xjs;
+                        if (cross && per && r + 1 == Bs && s + 1 == S)
+                            xjs = seg_load(&g.pub_head[(uint64_t)sg.first * W + (t + k - size)]) * vp.sc + vp.Lp;
+                        else
+                            xjs = xj * vp.sc;
+                        const double d0 = xj - xi, d1 = xjs - xis;
+                        if (k == 1 && d1 < gmin) {
+                            bad = 1;
+                            break;
+                        }
+                        if (d0 > q.rc && d1 > q.rc) break;
+                        if (sizeof(Real) == 8) {
+                            part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
+                        } else {
+                            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
+                                             phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
+                            const float y = tt - cc;
+                            const float z = sf + y;
+                            cc = (z - sf) - y;
+                            sf = z;
+                        }
+                    }
+                }
+                vpart = sizeof(Real) == 8 ? part : (double)sf - (double)cc;
+            }
+        }
+        const uint32_t par = (uint32_t)(sweep & 1u);
+        {
+            double v[2] = {dUs, vpart};
+            uint32_t k2[2] = {acc, bad};
+            seg_allreduce<256, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            if (tid == 0) {
+                put(par, 0, 0, 2);
+                put(par, 1, 1, 2);
+                g.part[seg_pidx(me, par, 2, 0)] = (double)k2[0];
+                g.part[seg_pidx(me, par, 3, 0)] = (double)k2[1];
+            }
+        }
+        sync_all();
+        gather(par, 0);
+        gather(par, 1);
+        gather_int(par, 2, 0);
+        gather_int(par, 3, 0);
+        U = U + tot[0];
+        acc = (uint32_t)tot[2];
+        acc_total += acc;
+        uint32_t accv = 0;
+        if (npt) {
+            ++n_vt;
+            if (vp.valid && tot[3] == 0.0 && vol_accept(tot[1], vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
+                for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
+                L = vp.Lp;
+                U = U + tot[1];
+                accv = 1;
+                ++n_av;
+                __syncthreads();
+                publish_all();
+                sync_nb();
+                refresh();
+            }
+        }
+
+        // drift guard (ECheck, src/jmmMCState.cpp:1965-2095), always FP64
+        if (eci && ((sweep + 1) % eci == 0)) {
+            double Up = 0.0;
+            if (act) {
+                for (uint32_t t = 0; t < size; ++t) {
+                    const uint32_t i = start + t;
+                    const uint32_t my = a0 + t;
+                    const double xi = c.R[my];
+                    const uint32_t kR = per ? w : min(w, N - 1 - i);
+                    for (uint32_t k = 1; k <= kR; ++k) {
+                        const double d = c.R[my + k + (t + k >= size ? gapR : 0)] - xi;
+                        if (d > q.rc) break;
+                        Up += phi64<KIND>(q, d);
+                    }
+                }
+            }
+            double v[1] = {Up};
+            uint32_t k1[1] = {0};
+            seg_allreduce<256, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            if (tid == 0) put(par, 4, 0, 1);
+            sync_all();
+            gather(par, 4);
+            const double dr = fabs(tot[4] - U);
+            if (dr > mdrift) mdrift = dr;
+            U = tot[4];
+            ++n_rc;
+        }
+
+        if (s == 0 && tid == 0) {
+            const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
+            a.sU[row] = U;
+            a.sL[row] = L;
+            a.sA[row] = make_uint2(acc, accv);
+        }
+    }
+
+    // counters: segment totals, summed by segment 0, which alone writes the
+    // chain's state
+    {
+        double vz[1] = {0.0};
+        uint32_t k4[4] = {n_tr, n_wall, n_core, n_terms};
+        seg_allreduce<256, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        const uint32_t par = (uint32_t)(sweep & 1u);
+        if (tid == 0)
+            for (uint32_t m = 0; m < 4; ++m) g.part[seg_pidx(me, par, 5, m)] = (double)k4[m];
+        sync_all();
+        if (s == 0 && tid == 0) {
+            uint64_t tt[4];
+            for (uint32_t m = 0; m < 4; ++m) {
+                double v = 0.0;
+                for (uint32_t j = 0; j < S; ++j) v += seg_load(&g.part[seg_pidx(sg.first + j, par, 5, m)]);
+                tt[m] = (uint64_t)v;
+            }
+            DevState &o = a.states[ic];
+            o.L = L;
+            o.U = U;
+            o.sweep = sweep;
+            o.cnt[0] += tt[0];
+            o.cnt[1] += acc_total;
+            o.cnt[2] += tt[1];
+            o.cnt[3] += tt[2];
+            o.cnt[4] += tt[3];
+            o.cnt[5] += n_vt;
+            o.cnt[6] += n_av;
+            o.cnt[7] += n_rc;
+            o.max_drift = mdrift;
+        }
+    }
+    double *xw = a.x + ch.x_off;
+    for (uint32_t t = 0; t < size; ++t) xw[start + t] = c.R[a0 + t];
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+k_sweep_seg(SegArgs g)
+{
+    extern __shared__ double smem[];
+    __shared__ BatchedScratch<256> sc;
+    const uint32_t soff = g.a.ctrl->sample_off;
+    const DevSeg sg = g.segs[blockIdx.x];
+    switch (g.a.chains[sg.chain].kind) {
+    case POT_LJ: run_segment<Real, POT_LJ>(g, sg, smem, sc, soff); break;
+    case POT_HARMONIC: run_segment<Real, POT_HARMONIC>(g, sg, smem, sc, soff); break;
+    case POT_HRSW: run_segment<Real, POT_HRSW>(g, sg, smem, sc, soff); break;
+    default: run_segment<Real, POT_YUKAWA>(g, sg, smem, sc, soff); break;
+    }
+}
+
 // --------------------------------------------------------------- plain form
 
 template <int KIND, bool GX>
@@ -1295,6 +1726,18 @@ hipError_t launch_rescale(const DevChain *chains, const DevState *st, double *x,
 {
     hipLaunchKernelGGL(k_rescale, dim3(n_chains * parts), dim3(256), 0, s, chains, st, x, Lnew, parts);
     return hipGetLastError();
+}
+
+const void *segmented_kernel(int fp32)
+{
+    return fp32 ? (const void *)k_sweep_seg<float> : (const void *)k_sweep_seg<double>;
+}
+
+hipError_t launch_segmented(const SegArgs &a, int fp32, size_t lds, hipStream_t s)
+{
+    void *args[] = {(void *)&a};
+    return hipLaunchCooperativeKernel(segmented_kernel(fp32), dim3(a.n_segs), dim3(256), args,
+                                      (unsigned int)lds, s);
 }
 
 hipError_t launch_hist(const HistArgs &a, hipStream_t s)
