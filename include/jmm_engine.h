@@ -92,7 +92,10 @@ typedef struct jmm_engine_opts {
                                  (0 = the device's LDS per workgroup) is split into segments of
                                  consecutive blocks, one workgroup each, that synchronise per
                                  phase; a chain that cannot be split runs from HBM */
-    uint32_t reserved[3];
+    uint32_t seg_lanes;       /* split chains: lanes per block, 1 or 2 (two lanes share each
+                                 trial, one per side; 0 = 1); results are the same bits
+                                 either way */
+    uint32_t reserved[2];
 } jmm_engine_opts;
 
 /* One per chain per sweep (the per-sweep form of the reference's per-trial
@@ -143,6 +146,8 @@ typedef struct jmm_layout {
     uint32_t mode;       /* JMM_MODE_* */
     uint32_t segments;   /* workgroups the chain spans (1 unless split) */
     uint64_t lds_bytes;  /* LDS image of the chain, or of one segment when split */
+    uint32_t lanes;      /* lanes per block (2 when split chains share trials over two lanes) */
+    uint32_t reserved;
 } jmm_layout;
 
 typedef struct jmm_engine jmm_engine;

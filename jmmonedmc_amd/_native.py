@@ -44,7 +44,7 @@ class EngineOpts(C.Structure):
         ("form", C.c_int32), ("device", C.c_int32), ("sample_capacity", C.c_uint32),
         ("block_sweeps", C.c_uint32), ("wg_threads", C.c_uint32), ("use_graph", C.c_uint32),
         ("log_accepts", C.c_uint32), ("lds_budget", C.c_uint32), ("seg_lds", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("seg_lanes", C.c_uint32), ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -64,7 +64,8 @@ MODES = {MODE_LDS: "lds", MODE_HBM: "hbm", MODE_SPLIT: "split"}
 
 class Layout(C.Structure):
     _fields_ = [("B", C.c_uint32), ("w", C.c_uint32), ("mode", C.c_uint32),
-                ("segments", C.c_uint32), ("lds_bytes", C.c_uint64)]
+                ("segments", C.c_uint32), ("lds_bytes", C.c_uint64), ("lanes", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class Counters(C.Structure):

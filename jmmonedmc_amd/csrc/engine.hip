@@ -246,6 +246,7 @@ struct jmm_engine {
     std::vector<jmm_layout> layout;  // external index
     uint32_t n_segs = 0, seg_wgmax = 1, seg_ctrl = 0, n_ctrl = 0;
     size_t seg_lds_bytes = 0;
+    int seg_tw = 256, seg_lpb = 1;  // split chains: workgroup width, lanes per block
     DevSeg *d_segs = nullptr;
     double *d_pub_head = nullptr, *d_pub_tail = nullptr, *d_part = nullptr;
     unsigned long long *d_ctr = nullptr;
@@ -311,7 +312,8 @@ int launch_block(jmm_engine *e, uint32_t m)
         g.err = e->d_seg_err;
         g.wgmax = e->seg_wgmax;
         g.n_segs = e->n_segs;
-        HIPCHK(launch_segmented(g, e->form == JMM_FORM_BATCHED_FP32, e->seg_lds_bytes, e->stream));
+        HIPCHK(launch_segmented(g, e->form == JMM_FORM_BATCHED_FP32, e->seg_tw, e->seg_lpb,
+                                e->seg_lds_bytes, e->stream));
     }
     return 0;
 }
@@ -567,6 +569,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         l.B = ch[c].B;
         l.w = ch[c].w;
         l.segments = 1;
+        l.lanes = 1;
         if (e->form == JMM_FORM_PLAIN_FP64) {
             l.B = 1;
             l.lds_bytes = sizeof(double) * specs[c].N;
@@ -682,7 +685,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     // segmented chains: one cooperative launch, one workgroup per segment
     {
         std::vector<DevSeg> segv;
-        uint32_t wgmax = 1;
+        uint32_t wgmax = 1, bsmax = 1;
         size_t img = 0;
         for (int k = 0; k < n_chains; ++k) {
             const uint32_t S = segS[order[k]];
@@ -699,17 +702,23 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
                 segv.push_back(g);
             }
             wgmax = std::max(wgmax, d.wg);
+            bsmax = std::max(bsmax, d.B / S);
             img = std::max(img, 2 * (size_t)d.wg + (size_t)(d.B / S) * d.Sp);
         }
         if (!segv.empty()) {
             const int fp32 = e->form == JMM_FORM_BATCHED_FP32;
+            if (o.seg_lanes > 2) return bail(fail(JMM_EINVAL, "jmm_create: seg_lanes must be 0, 1 or 2"));
+            e->seg_lpb = o.seg_lanes ? (int)o.seg_lanes : 1;
+            e->seg_tw = e->seg_lpb * bsmax > 256 ? 512 : 256;
+            for (int c = 0; c < n_chains; ++c)
+                if (segS[c]) e->layout[c].lanes = (uint32_t)e->seg_lpb;
             e->n_segs = (uint32_t)segv.size();
             e->seg_wgmax = wgmax;
             e->seg_lds_bytes = img * sizeof(double);
-            if (int rc = set_lds_attr(segmented_kernel(fp32), e->seg_lds_bytes)) return bail(rc);
+            const void *fn = segmented_kernel(fp32, e->seg_tw, e->seg_lpb);
+            if (int rc = set_lds_attr(fn, e->seg_lds_bytes)) return bail(rc);
             int occ = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, segmented_kernel(fp32), 256,
-                                                                e->seg_lds_bytes));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, e->seg_tw, e->seg_lds_bytes));
             if ((uint64_t)occ * (uint64_t)n_cu < e->n_segs)
                 return bail(fail(JMM_EINVAL, "jmm_create: split chains need %u workgroups resident at once, "
                                              "the device holds %d", e->n_segs, occ * n_cu));

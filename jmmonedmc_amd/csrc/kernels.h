@@ -139,8 +139,9 @@ hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, h
 hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s);
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s);
 hipError_t launch_hist(const HistArgs &a, hipStream_t s);
-const void *segmented_kernel(int fp32);
-hipError_t launch_segmented(const SegArgs &a, int fp32, size_t lds, hipStream_t s);
+// tw: workgroup width (256 or 512); lpb: lanes per block (1 or 2; 512 only with 2)
+const void *segmented_kernel(int fp32, int tw, int lpb);
+hipError_t launch_segmented(const SegArgs &a, int fp32, int tw, int lpb, size_t lds, hipStream_t s);
 hipError_t launch_rescale(const DevChain *chains, const DevState *st, double *x, const double *Lnew,
                           uint32_t n_chains, uint32_t parts, hipStream_t s);
 }  // namespace jmm

@@ -523,13 +523,21 @@ __device__ __forceinline__ void fill_ghosts(const LdsChain &c, uint32_t r, doubl
 // without branches, Kahan-compensated side sums; the loop-exit test runs on
 // the FP32 roundings (it only ends the walk, a pair it keeps adds zero).  Returns the decision; dE receives
 // the energy change.
-template <typename Real, int KIND>
+//
+// PAIR = false: one lane walks both sides.  PAIR = true: two adjacent lanes
+// share the trial, side 0 (even lane) walking the left partners and side 1
+// the right; each forms its side's sum exactly as the one-lane walk does
+// (same terms, same order; a side's FP32 sum stops changing when its walk
+// ends), then the two sums are exchanged and added.  Addition commutes, so
+// both lanes hold the one-lane result bit for bit.  Both lanes of a pair
+// must be active together.
+template <typename Real, int KIND, bool PAIR = false>
 __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, double gmin,
                                           double beta, float betaf, uint32_t i, uint32_t t,
                                           uint32_t size, uint32_t my, uint32_t gapL,
                                           uint32_t gapR, double L, double xi, double xn, double md,
                                           double u, double &dE, uint32_t &terms, uint32_t &n_wall,
-                                          uint32_t &n_core)
+                                          uint32_t &n_core, uint32_t side = 0)
 {
     const double *X = c.R;
     // rejections by the walls and the hard core are flags, not early exits:
@@ -545,6 +553,50 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     const bool go = !wall && !core;
     const uint32_t kL = c.per ? c.w : min(c.w, i);
     const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+    if (PAIR) {
+        // one side per lane: partner k at my - k (left) or my + k (right),
+        // separations oriented as in the two-sided walk
+        const bool left = side == 0;
+        const uint32_t kS = left ? kL : kR;
+        bool live = go && kS >= 1;
+        if (sizeof(Real) == 8) {
+            double dS = 0.0;
+#pragma unroll 1
+            for (uint32_t k = 1; __any(live); ++k) {
+                const uint32_t jl = my - k - (k > t ? gapL : 0), jr = my + k + (t + k >= size ? gapR : 0);
+                const double xj = X[live ? (left ? jl : jr) : my];
+                const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
+                const bool in = live && !(o > q.rc && nn > q.rc);
+                const double tt = phi64<KIND>(q, nn) - phi64<KIND>(q, o);
+                dS += in ? tt : 0.0;
+                terms += in ? 2u : 0u;
+                live = in && k < kS;
+            }
+            dE = dS + __shfl_xor(dS, 1);
+            const double bf = exp(-dE * beta);
+            return go && (dE <= 0.0 || bf > u);
+        }
+        float sS = 0.0f, cS = 0.0f;
+#pragma unroll 1
+        for (uint32_t k = 1; __any(live); ++k) {
+            const uint32_t jl = my - k - (k > t ? gapL : 0), jr = my + k + (t + k >= size ? gapR : 0);
+            const double xj = X[live ? (left ? jl : jr) : my];
+            const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
+            const float of = (float)o, nf = (float)nn;
+            const bool in = live && !(of > q.rcf && nf > q.rcf);
+            const float tt = phi32<KIND>(q, nf, inside64<KIND>(q, nn)) - phi32<KIND>(q, of, inside64<KIND>(q, o));
+            const float y = tt - cS;
+            const float uu = sS + y;
+            cS = in ? (uu - sS) - y : cS;
+            sS = in ? uu : sS;
+            terms += in ? 2u : 0u;
+            live = in && k < kS;
+        }
+        const double mine = (double)sS - (double)cS;
+        dE = mine + __shfl_xor(mine, 1);
+        const float bf = fexp(-betaf * (float)dE);
+        return go && (dE <= 0.0 || (double)bf > u);
+    }
     bool liveL = go && kL >= 1, liveR = go && kR >= 1;
     if (sizeof(Real) == 8) {
         double dEL = 0.0, dER = 0.0;
@@ -585,14 +637,15 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
                          phi32<KIND>(q, oLf, inside64<KIND>(q, oL));
         const float tR = phi32<KIND>(q, nRf, inside64<KIND>(q, nR)) -
                          phi32<KIND>(q, oRf, inside64<KIND>(q, oR));
-        const float yL = (inL ? tL : 0.0f) - cL;
+        // compensated sums; a side whose walk has ended keeps its sum
+        const float yL = tL - cL;
         const float uL = sL + yL;
-        cL = (uL - sL) - yL;
-        sL = uL;
-        const float yR = (inR ? tR : 0.0f) - cR;
+        cL = inL ? (uL - sL) - yL : cL;
+        sL = inL ? uL : sL;
+        const float yR = tR - cR;
         const float uR = sR + yR;
-        cR = (uR - sR) - yR;
-        sR = uR;
+        cR = inR ? (uR - sR) - yR : cR;
+        sR = inR ? uR : sR;
         terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
         liveL = inL && k < kL;
         liveR = inR && k < kR;
@@ -938,15 +991,23 @@ __device__ __forceinline__ uint64_t seg_pidx(uint32_t j, uint32_t par, uint32_t 
 // and the first block reads the left neighbour's last-block particles
 // p >= size - wg + t, which the neighbour (at most K phases ahead) has not
 // updated yet in this sweep.  So between two syncs every halo value read is
-// the one the sequential schedule reads.  Sync points, the same in every
+// the one the sequential schedule reads.
+//
+// LPB = 2 gives every block two lanes (adjacent; the even lane owns the
+// block's state): they share each displacement trial, one walking the left
+// partners and one the right (trial_lds with PAIR), which halves the walk a
+// lane does and doubles the waves per CU when LDS limits the workgroups.
+// Everything else (loads, stores, the box-length and recompute sums) is the
+// even lane's, and the sums are combined per 64 blocks as with one lane per
+// block, so LPB changes no result bit.  Sync points, the same in every
 // segment of a chain: the initial publication (neighbours); phases
 // K-1, 2K-1, ... and the last phase (neighbours); every sweep's partial sums
 // (all segments); a box-length acceptance (neighbours); a recompute (all);
 // the final counters (all).  Partial sums are double-buffered by sweep
 // parity: a segment is at most one all-segment point ahead of another.
-template <typename Real, int KIND>
+template <typename Real, int KIND, int TW, int LPB>
 __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
-                            BatchedScratch<256> &sc, uint32_t soff)
+                            BatchedScratch<TW> &sc, uint32_t soff)
 {
     const SweepArgs &a = g.a;
     const uint32_t tid = threadIdx.x;
@@ -967,7 +1028,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
     const uint32_t ln = sg.first + (s + S - 1) % S, rn = sg.first + (s + 1) % S;
     const uint32_t W = g.wgmax;
     const uint32_t K = qb > wg ? qb - wg : 1u;
-    const uint32_t nwv = Bs >= 64 ? Bs / 64 : 1u;  // wave totals per segment and field
+    const uint32_t nwv = Bs >= 64 ? Bs / 64 : 1u;  // 64-block totals per segment and field
 
     LdsChain c;
     c.R = smem;
@@ -980,22 +1041,25 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
     c.rem = rem;
     c.per = per;
 
-    const bool act = tid < Bs;
-    const uint32_t r = tid;
+    const uint32_t r = LPB == 2 ? tid >> 1 : tid;
+    const uint32_t side = LPB == 2 ? (tid & 1u) : 0u;
+    const bool own = side == 0;  // the lane that owns the block's state
+    const bool act = r < Bs;
     const uint32_t b = s * Bs + r;
     const uint32_t size = act ? qb + (b < rem ? 1u : 0u) : 0u;
     const uint32_t start = act ? b * qb + (b < rem ? b : rem) : 0u;
     const uint32_t a0 = wg + r * Sp;
     const uint32_t gapR = Sp - size;
     const uint32_t gapL = (r == 0 || !act) ? 0u : Sp - (qb + (b - 1 < rem ? 1u : 0u));
-    const uint32_t seg = act ? Bs : 1u;
+    const uint32_t seg = act ? Bs * LPB : 1u;
     // the segment's last block
     const uint32_t bl = s * Bs + Bs - 1;
     const uint32_t size_last = qb + (bl < rem ? 1u : 0u);
     const uint32_t a0_last = wg + (Bs - 1) * Sp;
 
     const double *xg = a.x + ch.x_off;
-    for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = xg[start + t];
+    if (own)
+        for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = xg[start + t];
     const DevState st = a.states[ic];
     double L = st.L, U = st.U, mdrift = st.max_drift;
     uint64_t sweep = st.sweep;
@@ -1003,13 +1067,13 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
     __syncthreads();
 
     auto publish_all = [&]() {
-        for (uint32_t m = tid; m < wg; m += 256) {
+        for (uint32_t m = tid; m < wg; m += TW) {
             g.pub_head[(uint64_t)me * W + m] = c.R[wg + m];
             g.pub_tail[(uint64_t)me * W + m] = c.R[a0_last + size_last - wg + m];
         }
     };
     auto refresh = [&]() {
-        for (uint32_t m = tid; m < wg; m += 256) {
+        for (uint32_t m = tid; m < wg; m += TW) {
             if (hasLn) {
                 const double v = seg_load(&g.pub_tail[(uint64_t)ln * W + m]);
                 c.R[m] = (per && s == 0) ? v - L : v;
@@ -1043,10 +1107,14 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         }
         __syncthreads();
     };
-    // this segment's wave totals of slot m of the last nd-wide seg_allreduce
-    // (thread 0)
+    // this segment's totals per 64 blocks of slot m of the last nd-wide
+    // seg_allreduce (thread 0): wave totals, or pairs of them with two lanes
+    // per block (the top level of the 64-block tree)
     auto put = [&](uint32_t par, uint32_t f, uint32_t m, uint32_t nd) {
-        for (uint32_t wv = 0; wv < nwv; ++wv) g.part[seg_pidx(me, par, f, wv)] = sc.rd[wv * nd + m];
+        for (uint32_t wv = 0; wv < nwv; ++wv)
+            g.part[seg_pidx(me, par, f, wv)] =
+                (LPB == 2 && Bs >= 64) ? sc.rd[(2 * wv) * nd + m] + sc.rd[(2 * wv + 1) * nd + m]
+                                       : sc.rd[wv * nd + m];
     };
     // Field f summed over the chain's blocks in the order of seg_allreduce
     // over B = S*Bs lanes: pairwise trees within 64-lane groups, the group
@@ -1124,17 +1192,19 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
                 const double xn = xi + md;
                 ++n_tr;
                 double dE = 0.0;
-                const bool accd = trial_lds<Real, KIND>(c, q, gmin, beta, betaf, i, t, size, my,
-                                                        gapL, gapR, L, xi, xn, md, u, dE, n_terms,
-                                                        n_wall, n_core);
-                const double xs = accd ? xn : xi;
-                c.R[my] = xs;
-                dUs += accd ? dE : 0.0;
-                acc += accd ? 1u : 0u;
-                if (a.acclog) a.acclog[log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
-                // boundary particles the neighbouring segments read
-                if (r == 0 && t < wg) g.pub_head[(uint64_t)me * W + t] = xs;
-                if (r + 1 == Bs && t + wg >= size) g.pub_tail[(uint64_t)me * W + (t + wg - size)] = xs;
+                const bool accd = trial_lds<Real, KIND, LPB == 2>(c, q, gmin, beta, betaf, i, t, size, my,
+                                                                  gapL, gapR, L, xi, xn, md, u, dE,
+                                                                  n_terms, n_wall, n_core, side);
+                if (own) {
+                    const double xs = accd ? xn : xi;
+                    c.R[my] = xs;
+                    dUs += accd ? dE : 0.0;
+                    acc += accd ? 1u : 0u;
+                    if (a.acclog) a.acclog[log_off + (uint64_t)(soff + sw) * N + i] = accd ? 1 : 0;
+                    // boundary particles the neighbouring segments read
+                    if (r == 0 && t < wg) g.pub_head[(uint64_t)me * W + t] = xs;
+                    if (r + 1 == Bs && t + wg >= size) g.pub_tail[(uint64_t)me * W + (t + wg - size)] = xs;
+                }
             }
             if ((t + 1) % K == 0 || t + 1 == smax) {
                 sync_nb();
@@ -1151,7 +1221,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         double vpart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
-            if (vp.valid && act) {
+            if (vp.valid && act && own) {
                 double part = 0.0;
                 float sf = 0.0f, cc = 0.0f;
 #pragma unroll 1
@@ -1194,7 +1264,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         {
             double v[2] = {dUs, vpart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<256, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<TW, 2, 2>(v, k2, seg, sc.rd, sc.ri);
             if (tid == 0) {
                 put(par, 0, 0, 2);
                 put(par, 1, 1, 2);
@@ -1214,7 +1284,8 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         if (npt) {
             ++n_vt;
             if (vp.valid && tot[3] == 0.0 && vol_accept(tot[1], vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
-                for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
+                if (own)
+                    for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
                 L = vp.Lp;
                 U = U + tot[1];
                 accv = 1;
@@ -1229,7 +1300,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         // drift guard (ECheck, src/jmmMCState.cpp:1965-2095), always FP64
         if (eci && ((sweep + 1) % eci == 0)) {
             double Up = 0.0;
-            if (act) {
+            if (act && own) {
                 for (uint32_t t = 0; t < size; ++t) {
                     const uint32_t i = start + t;
                     const uint32_t my = a0 + t;
@@ -1244,7 +1315,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
             }
             double v[1] = {Up};
             uint32_t k1[1] = {0};
-            seg_allreduce<256, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<TW, 1, 1>(v, k1, seg, sc.rd, sc.ri);
             if (tid == 0) put(par, 4, 0, 1);
             sync_all();
             gather(par, 4);
@@ -1266,8 +1337,8 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
     // chain's state
     {
         double vz[1] = {0.0};
-        uint32_t k4[4] = {n_tr, n_wall, n_core, n_terms};
-        seg_allreduce<256, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        uint32_t k4[4] = {own ? n_tr : 0u, own ? n_wall : 0u, own ? n_core : 0u, n_terms};
+        seg_allreduce<TW, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
         const uint32_t par = (uint32_t)(sweep & 1u);
         if (tid == 0)
             for (uint32_t m = 0; m < 4; ++m) g.part[seg_pidx(me, par, 5, m)] = (double)k4[m];
@@ -1295,22 +1366,23 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         }
     }
     double *xw = a.x + ch.x_off;
-    for (uint32_t t = 0; t < size; ++t) xw[start + t] = c.R[a0 + t];
+    if (own)
+        for (uint32_t t = 0; t < size; ++t) xw[start + t] = c.R[a0 + t];
 }
 
-template <typename Real>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+template <typename Real, int TW, int LPB>
+__global__ void __launch_bounds__(TW) __attribute__((amdgpu_waves_per_eu(1)))
 k_sweep_seg(SegArgs g)
 {
     extern __shared__ double smem[];
-    __shared__ BatchedScratch<256> sc;
+    __shared__ BatchedScratch<TW> sc;
     const uint32_t soff = g.a.ctrl->sample_off;
     const DevSeg sg = g.segs[blockIdx.x];
     switch (g.a.chains[sg.chain].kind) {
-    case POT_LJ: run_segment<Real, POT_LJ>(g, sg, smem, sc, soff); break;
-    case POT_HARMONIC: run_segment<Real, POT_HARMONIC>(g, sg, smem, sc, soff); break;
-    case POT_HRSW: run_segment<Real, POT_HRSW>(g, sg, smem, sc, soff); break;
-    default: run_segment<Real, POT_YUKAWA>(g, sg, smem, sc, soff); break;
+    case POT_LJ: run_segment<Real, POT_LJ, TW, LPB>(g, sg, smem, sc, soff); break;
+    case POT_HARMONIC: run_segment<Real, POT_HARMONIC, TW, LPB>(g, sg, smem, sc, soff); break;
+    case POT_HRSW: run_segment<Real, POT_HRSW, TW, LPB>(g, sg, smem, sc, soff); break;
+    default: run_segment<Real, POT_YUKAWA, TW, LPB>(g, sg, smem, sc, soff); break;
     }
 }
 
@@ -1728,15 +1800,19 @@ hipError_t launch_rescale(const DevChain *chains, const DevState *st, double *x,
     return hipGetLastError();
 }
 
-const void *segmented_kernel(int fp32)
+const void *segmented_kernel(int fp32, int tw, int lpb)
 {
-    return fp32 ? (const void *)k_sweep_seg<float> : (const void *)k_sweep_seg<double>;
+    if (lpb == 2 && tw == 512)
+        return fp32 ? (const void *)k_sweep_seg<float, 512, 2> : (const void *)k_sweep_seg<double, 512, 2>;
+    if (lpb == 2)
+        return fp32 ? (const void *)k_sweep_seg<float, 256, 2> : (const void *)k_sweep_seg<double, 256, 2>;
+    return fp32 ? (const void *)k_sweep_seg<float, 256, 1> : (const void *)k_sweep_seg<double, 256, 1>;
 }
 
-hipError_t launch_segmented(const SegArgs &a, int fp32, size_t lds, hipStream_t s)
+hipError_t launch_segmented(const SegArgs &a, int fp32, int tw, int lpb, size_t lds, hipStream_t s)
 {
     void *args[] = {(void *)&a};
-    return hipLaunchCooperativeKernel(segmented_kernel(fp32), dim3(a.n_segs), dim3(256), args,
+    return hipLaunchCooperativeKernel(segmented_kernel(fp32, tw, lpb), dim3(a.n_segs), dim3(tw), args,
                                       (unsigned int)lds, s);
 }
 

@@ -76,7 +76,7 @@ class Engine:
 
     def __init__(self, chains, form="batched_fp32", device=0, sample_capacity=1024,
                  block_sweeps=100, wg_threads=0, use_graph=False, log_accepts=False,
-                 lds_budget=0, seg_lds=0):
+                 lds_budget=0, seg_lds=0, seg_lanes=0):
         self.chains = list(chains)
         self.n = len(self.chains)
         self.form = nat.FORMS[form] if isinstance(form, str) else int(form)
@@ -89,6 +89,7 @@ class Engine:
         o.wg_threads, o.use_graph, o.log_accepts, o.lds_budget = (wg_threads, int(use_graph),
                                                                   int(log_accepts), lds_budget)
         o.seg_lds = seg_lds
+        o.seg_lanes = seg_lanes
         h = C.c_void_p()
         check(lib().jmm_create(specs, self.n, C.byref(o), C.byref(h)), "jmm_create")
         self.h = h
@@ -183,10 +184,11 @@ class Engine:
 
     def layout(self, chain: int) -> dict:
         """Placement of a chain: blocks B, window w, memory mode ("lds",
-        "hbm" or "split"), workgroups spanned, LDS bytes."""
+        "hbm" or "split"), workgroups spanned, LDS bytes, lanes per block."""
         l = nat.Layout()
         check(lib().jmm_chain_layout(self.h, chain, C.byref(l)), "jmm_chain_layout")
-        return dict(B=l.B, w=l.w, mode=nat.MODES[l.mode], segments=l.segments, lds_bytes=l.lds_bytes)
+        return dict(B=l.B, w=l.w, mode=nat.MODES[l.mode], segments=l.segments, lds_bytes=l.lds_bytes,
+                    lanes=l.lanes)
 
     def all_counters(self) -> dict:
         """Every chain's counters in one call: dict of arrays indexed by chain."""

@@ -13,7 +13,7 @@ split chains are checked
 
 Cases cover periodic and wall boundaries, NVT and NPT (box-length moves
 across segment boundaries and across the periodic seam), the drift-guard
-recompute, and every potential."""
+recompute, every potential, and one or two lanes per block."""
 import math
 
 import numpy as np
@@ -73,9 +73,9 @@ def seg_lds_for(oracle, ch, S):
     return 8 * (2 * wg + (B // S) * Sp), B
 
 
-def run(chains, form, seg_lds=0, log=False):
+def run(chains, form, seg_lds=0, log=False, lanes=0):
     with Engine(chains, form=form, sample_capacity=SWEEPS, block_sweeps=BLOCK,
-                seg_lds=seg_lds, log_accepts=log) as e:
+                seg_lds=seg_lds, log_accepts=log, seg_lanes=lanes) as e:
         lay = [e.layout(c) for c in range(len(chains))]
         e.run(SWEEPS)
         logs = e.read_accept_log(SWEEPS) if log else None
@@ -108,9 +108,12 @@ def segment_counts(oracle, ch):
     return sorted(set(out))
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("form", ["batched_fp64", "batched_fp32"])
 @pytest.mark.parametrize("name", NAMES)
-def test_split_chain_equals_whole_chain(gpu, oracle, name, form):
+def test_split_chain_equals_whole_chain(gpu, oracle, name, form, lanes):
+    """lanes = 2: two lanes per block share every trial (one walks the left
+    partners, one the right)."""
     ch = split_case(name)
     # a whole chain in the same engine (its image stays under every seg_lds used)
     small = Chain(N=32, rho=0.5, T=1.0, seed=99, chain_id=99, **lj())
@@ -121,21 +124,22 @@ def test_split_chain_equals_whole_chain(gpu, oracle, name, form):
     assert counts, name
     for S in counts:
         seg_lds, B = seg_lds_for(oracle, ch, S)
-        print(f"[gpu] split {name} {form} B={B} S={S}", flush=True)
-        got = run(chains, form, seg_lds=seg_lds)
+        print(f"[gpu] split {name} {form} B={B} S={S} lanes={lanes}", flush=True)
+        got = run(chains, form, seg_lds=seg_lds, lanes=lanes)
         lay = got["layout"][1]
-        assert lay["mode"] == "split" and lay["segments"] == S, lay
+        assert lay["mode"] == "split" and lay["segments"] == S and lay["lanes"] == lanes, lay
         assert got["layout"][0]["mode"] == "lds"
         assert_same(got, whole)
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("name", NAMES)
-def test_split_chain_replays_oracle(gpu, oracle, name):
+def test_split_chain_replays_oracle(gpu, oracle, name, lanes):
     ch = split_case(name)
     S = segment_counts(oracle, ch)[-1]
     seg_lds, B = seg_lds_for(oracle, ch, S)
-    print(f"[gpu] split oracle {name} B={B} S={S}", flush=True)
-    got = run([ch], "batched_fp64", seg_lds=seg_lds, log=True)
+    print(f"[gpu] split oracle {name} B={B} S={S} lanes={lanes}", flush=True)
+    got = run([ch], "batched_fp64", seg_lds=seg_lds, log=True, lanes=lanes)
     assert got["layout"][0]["mode"] == "split"
     orc = oracle.Chain(oracle_spec(oracle, ch))
     _, olog = orc.sweeps(SWEEPS, log=True)
