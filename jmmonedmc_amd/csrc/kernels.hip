@@ -655,6 +655,74 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     return go && (dE <= 0.0 || (double)bf > u);
 }
 
+// Box-length trial (fav, src/jmmMCState.cpp:2161-2293), one lane's share:
+// the energy change of scaling the configuration by vp.sc, summed pair by
+// pair over the pairs (i, i+k) whose left member is in the lane's block, in
+// the order t = 0.., k = 1.. .  The walk over (particle, partner) is one
+// flat loop with per-lane state, so lanes at different particles advance
+// together without nested divergent loops; it adds the same terms in the
+// same order as the nested walk (particle t's walk ends at its first partner
+// beyond the cutoff from both scaled and unscaled positions, at k = kR, or
+// at an overlap of nearest neighbours, which ends the lane's walk and sets
+// `bad`).  The partner at k = 1 is the next particle, so its position and
+// scaled position are kept rather than read again.  `last`: the lane holds
+// the chain's last block of a periodic chain, whose partners past the end
+// are the first particles shifted by L; wrap(m) returns particle m's
+// unshifted position (scaled and shifted by Lp here).
+template <typename Real, int KIND, typename Wrap>
+__device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, double gmin,
+                                           const VolProp &vp, bool on, uint32_t start,
+                                           uint32_t size, uint32_t a0, uint32_t gapR, bool last,
+                                           Wrap wrap, uint32_t &bad)
+{
+    const double *X = c.R;
+    double part = 0.0;
+    float sf = 0.0f, cc = 0.0f;
+    bool run = on && size > 0;
+    uint32_t t = 0, k = 1;
+    double xi = X[run ? a0 : 0u];
+    double xis = xi * vp.sc;
+    double x1 = 0.0, x1s = 0.0;
+#pragma unroll 1
+    while (__any(run)) {
+        const uint32_t i = start + t;
+        const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+        const bool valid = run && k <= kR;
+        const bool cross = t + k >= size;
+        const double xj = X[valid ? a0 + t + k + (cross ? gapR : 0u) : 0u];
+        double xjs = xj * vp.sc;
+        if (valid && cross && last) xjs = wrap(t + k - size) * vp.sc + vp.Lp;
+        const double d0 = xj - xi, d1 = xjs - xis;
+        const bool isbad = valid && k == 1 && d1 < gmin;
+        const bool stop = valid && d0 > q.rc && d1 > q.rc;
+        const bool add = valid && !isbad && !stop;
+        if (sizeof(Real) == 8) {
+            const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
+            part = add ? part + tt : part;
+        } else {
+            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
+                             phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
+            const float y = tt - cc;
+            const float z = sf + y;
+            cc = add ? (z - sf) - y : cc;
+            sf = add ? z : sf;
+        }
+        bad |= isbad ? 1u : 0u;
+        x1 = k == 1 ? xj : x1;
+        x1s = k == 1 ? xjs : x1s;
+        // next partner, or the next particle (lanes no longer running keep
+        // stepping t; their reads are redirected and their terms dropped)
+        const bool nextp = !valid || stop || k == kR;
+        const uint32_t tn = nextp ? t + 1 : t;
+        run = run && !isbad && tn < size;
+        xi = nextp ? x1 : xi;
+        xis = nextp ? x1s : xis;
+        k = nextp ? 1u : k + 1;
+        t = tn;
+    }
+    return sizeof(Real) == 8 ? part : (double)sf - (double)cc;
+}
+
 template <typename Real, int KIND, int T>
 __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBundle &bd, double *smem,
                                             BatchedScratch<T> &sc, uint32_t soff)
@@ -773,42 +841,9 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         double vpart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
-            if (vp.valid && act) {
-                double part = 0.0;
-                float s = 0.0f, cc = 0.0f;
-#pragma unroll 1
-                for (uint32_t t = 0; t < size && !bad; ++t) {
-                    const uint32_t i = start + t;
-                    const uint32_t my = a0 + t;
-                    const double xi = c.R[my], xis = xi * vp.sc;
-                    const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
-#pragma unroll 1
-                    for (uint32_t k = 1; k <= kR; ++k) {
-                        const bool cross = t + k >= size;
-                        const double xj = c.R[my + k + (cross ? gapR : 0)];
-                        double xjs;
-                        if (cross && r + 1 == c.B) xjs = c.R[c.wg + (t + k - size)] * vp.sc + vp.Lp;
-                        else xjs = xj * vp.sc;
-                        const double d0 = xj - xi, d1 = xjs - xis;
-                        if (k == 1 && d1 < gmin) {
-                            bad = 1;
-                            break;
-                        }
-                        if (d0 > q.rc && d1 > q.rc) break;
-                        if (sizeof(Real) == 8) {
-                            part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
-                        } else {
-                            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
-                                             phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
-                            const float y = tt - cc;
-                            const float z = s + y;
-                            cc = (z - s) - y;
-                            s = z;
-                        }
-                    }
-                }
-                vpart = sizeof(Real) == 8 ? part : (double)s - (double)cc;
-            }
+            vpart = vol_lane<Real, KIND>(c, q, gmin, vp, vp.valid && act, start, size, a0, gapR,
+                                         c.per && r + 1 == c.B,
+                                         [&](uint32_t m) { return c.R[c.wg + m]; }, bad);
         }
         {
             double v[2] = {dUs, vpart};
@@ -1221,44 +1256,10 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         double vpart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
-            if (vp.valid && act && own) {
-                double part = 0.0;
-                float sf = 0.0f, cc = 0.0f;
-#pragma unroll 1
-                for (uint32_t t = 0; t < size && !bad; ++t) {
-                    const uint32_t i = start + t;
-                    const uint32_t my = a0 + t;
-                    const double xi = c.R[my], xis = xi * vp.sc;
-                    const uint32_t kR = per ? w : min(w, N - 1 - i);
-#pragma unroll 1
-                    for (uint32_t k = 1; k <= kR; ++k) {
-                        const bool cross = t + k >= size;
-                        const double xj = c.R[my + k + (cross ? gapR : 0)];
-                        double xjs;
-                        if (cross && per && r + 1 == Bs && s + 1 == S)
-                            xjs = seg_load(&g.pub_head[(uint64_t)sg.first * W + (t + k - size)]) * vp.sc + vp.Lp;
-                        else
-                            xjs = xj * vp.sc;
-                        const double d0 = xj - xi, d1 = xjs - xis;
-                        if (k == 1 && d1 < gmin) {
-                            bad = 1;
-                            break;
-                        }
-                        if (d0 > q.rc && d1 > q.rc) break;
-                        if (sizeof(Real) == 8) {
-                            part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
-                        } else {
-                            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
-                                             phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
-                            const float y = tt - cc;
-                            const float z = sf + y;
-                            cc = (z - sf) - y;
-                            sf = z;
-                        }
-                    }
-                }
-                vpart = sizeof(Real) == 8 ? part : (double)sf - (double)cc;
-            }
+            vpart = vol_lane<Real, KIND>(
+                c, q, gmin, vp, vp.valid && act && own, start, size, a0, gapR,
+                per && r + 1 == Bs && s + 1 == S,
+                [&](uint32_t m) { return seg_load(&g.pub_head[(uint64_t)sg.first * W + m]); }, bad);
         }
         const uint32_t par = (uint32_t)(sweep & 1u);
         {
