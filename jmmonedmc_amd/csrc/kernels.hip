@@ -106,10 +106,13 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
     return left + ((double)s - (double)c);
 }
 
-// Pair-by-pair energy change of scaling every coordinate by sc (box-length
-// trial), over the pairs whose left member is in [p0, p1).  Sets bad when a
-// scaled nearest pair falls inside the hard core.  Restates fav's trial
-// energy (src/jmmMCState.cpp:2175-2235) as a difference per pair.
+// Box-length trial energy over the pairs whose left member is in [p0, p1),
+// for a scaling of every coordinate by sc.  Sets bad when a scaled nearest
+// pair falls inside the hard core.  FP64 form: the energy change pair by
+// pair, restating fav's trial energy (src/jmmMCState.cpp:2175-2235) as a
+// difference per pair.  FP32 form: the scaled configuration's energy
+// sum phi32(s d) alone (the caller subtracts the running energy), walked to
+// the first partner beyond the cutoff at the scaled separation.
 template <typename Real, int KIND>
 __device__ __forceinline__ double scaled_delta(const Pot &q, double gmin, const double *X, int64_t N,
                                                bool per, double L, double Lp, double sc,
@@ -137,12 +140,12 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, double gmin, const 
                 bad = 1;
                 break;
             }
-            if (d0 > q.rc && d1 > q.rc) break;
             if (sizeof(Real) == 8) {
+                if (d0 > q.rc && d1 > q.rc) break;
                 part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             } else {
-                const float t = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
-                                phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
+                if (d1 > q.rc) break;
+                const float t = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
                 const float y = t - c;
                 const float tt = s + y;
                 c = (tt - s) - y;
@@ -410,7 +413,8 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
             uint32_t k2[2] = {acc, bad};
             seg_allreduce<T, 2, 2>(v, k2, seg, sc.rd, sc.ri);
             U = U + v[0];
-            vpart = v[1];
+            // FP32 form: the scaled energy minus the running energy
+            vpart = sizeof(Real) == 8 ? v[1] : v[1] - U;
             acc = k2[0];
             bad = k2[1];
         }
@@ -655,10 +659,14 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     return go && (dE <= 0.0 || (double)bf > u);
 }
 
-// Box-length trial (fav, src/jmmMCState.cpp:2161-2293), one lane's share:
-// the energy change of scaling the configuration by vp.sc, summed pair by
-// pair over the pairs (i, i+k) whose left member is in the lane's block, in
-// the order t = 0.., k = 1.. .  The walk over (particle, partner) is one
+// Box-length trial (fav, src/jmmMCState.cpp:2161-2293), one lane's share
+// over the pairs (i, i+k) whose left member is in the lane's block, in the
+// order t = 0.., k = 1.. : FP64 form, the energy change of scaling the
+// configuration by vp.sc pair by pair; FP32 form, the scaled
+// configuration's energy sum phi32(s d) alone, the caller subtracting the
+// running energy (half the pair evaluations; particle t's walk then ends at
+// its first partner beyond the cutoff at the scaled separation).  The walk
+// over (particle, partner) is one
 // flat loop with per-lane state, so lanes at different particles advance
 // together without nested divergent loops; it adds the same terms in the
 // same order as the nested walk (particle t's walk ends at its first partner
@@ -694,14 +702,13 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         if (valid && cross && last) xjs = wrap(t + k - size) * vp.sc + vp.Lp;
         const double d0 = xj - xi, d1 = xjs - xis;
         const bool isbad = valid && k == 1 && d1 < gmin;
-        const bool stop = valid && d0 > q.rc && d1 > q.rc;
+        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : d1 > q.rc);
         const bool add = valid && !isbad && !stop;
         if (sizeof(Real) == 8) {
             const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             part = add ? part + tt : part;
         } else {
-            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1)) -
-                             phi32<KIND>(q, (float)d0, inside64<KIND>(q, d0));
+            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
             const float y = tt - cc;
             const float z = sf + y;
             cc = add ? (z - sf) - y : cc;
@@ -850,7 +857,8 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
             uint32_t k2[2] = {acc, bad};
             seg_allreduce<T, 2, 2>(v, k2, seg, sc.rd, sc.ri);
             U = U + v[0];
-            vpart = v[1];
+            // FP32 form: the scaled energy minus the running energy
+            vpart = sizeof(Real) == 8 ? v[1] : v[1] - U;
             acc = k2[0];
             bad = k2[1];
         }
@@ -1284,11 +1292,13 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         uint32_t accv = 0;
         if (npt) {
             ++n_vt;
-            if (vp.valid && tot[3] == 0.0 && vol_accept(tot[1], vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
+            // FP32 form: the scaled energy minus the running energy
+            const double dUv = sizeof(Real) == 8 ? tot[1] : tot[1] - U;
+            if (vp.valid && tot[3] == 0.0 && vol_accept(dUv, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
                 if (own)
                     for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
                 L = vp.Lp;
-                U = U + tot[1];
+                U = U + dUv;
                 accv = 1;
                 ++n_av;
                 __syncthreads();
