@@ -168,9 +168,10 @@ JMM_HD float phi32(const Pot &q, float d, bool in)
 {
     float v;
     if (KIND == POT_LJ) {
+        // 4 eps s6 (s6 - 1) - shift as s6 (4 eps s6 - 4 eps) - shift, two FMAs
         const float ir2 = frcp(d * d);
         const float s6 = ir2 * ir2 * ir2 * q.sig6f;
-        v = q.e4f * s6 * (s6 - 1.0f) - q.shiftf;
+        v = fmaf(s6, fmaf(q.e4f, s6, -q.e4f), -q.shiftf);
     } else if (KIND == POT_HARMONIC) {
         const float m = d - 1.0f;
         v = d <= 0.0f ? 10e10f : m * m;

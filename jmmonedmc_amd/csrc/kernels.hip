@@ -516,6 +516,16 @@ __device__ __forceinline__ void fill_ghosts(const LdsChain &c, uint32_t r, doubl
     }
 }
 
+// The walk's keep-going test for one partner: not beyond the cutoff from
+// both the old and the new position (FP64, the FP64 form's rule).  For the
+// potentials whose inside test is d <= rc it is the inside flags' or.
+template <int KIND>
+__device__ __forceinline__ bool walk_in(const Pot &q, double o, double n, bool io, bool in)
+{
+    if (KIND == POT_LJ || KIND == POT_YUKAWA) return io || in;
+    return !(o > q.rc && n > q.rc);
+}
+
 // One displacement trial of particle i (block position t of a block of
 // length size, LDS address my) by md.  Same decisions as orc_trial_with: wall
 // test, hard core against the nearest partners (FP64), then the partner sums
@@ -525,8 +535,11 @@ __device__ __forceinline__ void fill_ghosts(const LdsChain &c, uint32_t r, doubl
 // FP32 form: separations formed in FP64, cutoff decided in FP64 (the FP64
 // set-up and recompute energies count the same pairs), pair energies in FP32
 // without branches, Kahan-compensated side sums; the loop-exit test runs on
-// the FP32 roundings (it only ends the walk, a pair it keeps adds zero).  Returns the decision; dE receives
-// the energy change.
+// the separations in FP64 as in the FP64 form.  Partner k's address is read
+// on every lane of the wave, live or not: for k <= w it lies in the chain's
+// image (ghosts or halos are wg >= w wide, blocks longer than w), and a lane
+// that is not live drops the value.  Returns the decision; dE receives the
+// energy change.
 //
 // PAIR = false: one lane walks both sides.  PAIR = true: two adjacent lanes
 // share the trial, side 0 (even lane) walking the left partners and side 1
@@ -568,7 +581,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 #pragma unroll 1
             for (uint32_t k = 1; __any(live); ++k) {
                 const uint32_t jl = my - k - (k > t ? gapL : 0), jr = my + k + (t + k >= size ? gapR : 0);
-                const double xj = X[live ? (left ? jl : jr) : my];
+                const double xj = X[left ? jl : jr];
                 const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
                 const bool in = live && !(o > q.rc && nn > q.rc);
                 const double tt = phi64<KIND>(q, nn) - phi64<KIND>(q, o);
@@ -584,11 +597,11 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 #pragma unroll 1
         for (uint32_t k = 1; __any(live); ++k) {
             const uint32_t jl = my - k - (k > t ? gapL : 0), jr = my + k + (t + k >= size ? gapR : 0);
-            const double xj = X[live ? (left ? jl : jr) : my];
+            const double xj = X[left ? jl : jr];
             const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
-            const float of = (float)o, nf = (float)nn;
-            const bool in = live && !(of > q.rcf && nf > q.rcf);
-            const float tt = phi32<KIND>(q, nf, inside64<KIND>(q, nn)) - phi32<KIND>(q, of, inside64<KIND>(q, o));
+            const bool io = inside64<KIND>(q, o), inn = inside64<KIND>(q, nn);
+            const bool in = live && walk_in<KIND>(q, o, nn, io, inn);
+            const float tt = phi32<KIND>(q, (float)nn, inn) - phi32<KIND>(q, (float)o, io);
             const float y = tt - cS;
             const float uu = sS + y;
             cS = in ? (uu - sS) - y : cS;
@@ -606,8 +619,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         double dEL = 0.0, dER = 0.0;
 #pragma unroll 1
         for (uint32_t k = 1; __any(liveL || liveR); ++k) {
-            const double xjL = X[liveL ? my - k - (k > t ? gapL : 0) : my];
-            const double xjR = X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
+            const double xjL = X[my - k - (k > t ? gapL : 0)];
+            const double xjR = X[my + k + (t + k >= size ? gapR : 0)];
             const double oL = xi - xjL, nL = xn - xjL;
             const double oR = xjR - xi, nR = xjR - xn;
             const bool inL = liveL && !(oL > q.rc && nL > q.rc);
@@ -627,20 +640,20 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
 #pragma unroll 1
     for (uint32_t k = 1; __any(liveL || liveR); ++k) {
-        const double xjL = X[liveL ? my - k - (k > t ? gapL : 0) : my];
-        const double xjR = X[liveR ? my + k + (t + k >= size ? gapR : 0) : my];
-        // separations are FP64 differences; the cutoff decision is made on
+        const double xjL = X[my - k - (k > t ? gapL : 0)];
+        const double xjR = X[my + k + (t + k >= size ? gapR : 0)];
+        // separations are FP64 differences; the cutoff decisions are made on
         // them (the FP64 energies of set-up and recompute count the same
-        // pairs), the pair energies on their FP32 roundings
+        // pairs, and the walk ends where the FP64 form's does), the pair
+        // energies on their FP32 roundings
         const double oL = xi - xjL, nL = xn - xjL;
         const double oR = xjR - xi, nR = xjR - xn;
-        const float oLf = (float)oL, nLf = (float)nL, oRf = (float)oR, nRf = (float)nR;
-        const bool inL = liveL && !(oLf > q.rcf && nLf > q.rcf);
-        const bool inR = liveR && !(oRf > q.rcf && nRf > q.rcf);
-        const float tL = phi32<KIND>(q, nLf, inside64<KIND>(q, nL)) -
-                         phi32<KIND>(q, oLf, inside64<KIND>(q, oL));
-        const float tR = phi32<KIND>(q, nRf, inside64<KIND>(q, nR)) -
-                         phi32<KIND>(q, oRf, inside64<KIND>(q, oR));
+        const bool ioL = inside64<KIND>(q, oL), inL64 = inside64<KIND>(q, nL);
+        const bool ioR = inside64<KIND>(q, oR), inR64 = inside64<KIND>(q, nR);
+        const bool inL = liveL && walk_in<KIND>(q, oL, nL, ioL, inL64);
+        const bool inR = liveR && walk_in<KIND>(q, oR, nR, ioR, inR64);
+        const float tL = phi32<KIND>(q, (float)nL, inL64) - phi32<KIND>(q, (float)oL, ioL);
+        const float tR = phi32<KIND>(q, (float)nR, inR64) - phi32<KIND>(q, (float)oR, ioR);
         // compensated sums; a side whose walk has ended keeps its sum
         const float yL = tL - cL;
         const float uL = sL + yL;
