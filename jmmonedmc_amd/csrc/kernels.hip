@@ -578,8 +578,10 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         bool live = go && kS >= 1;
         if (sizeof(Real) == 8) {
             double dS = 0.0;
+            uint32_t k = 1;
+            if (__any(live)) {
 #pragma unroll 1
-            for (uint32_t k = 1; __any(live); ++k) {
+            do {
                 const uint32_t jl = my - k - (k > t ? gapL : 0), jr = my + k + (t + k >= size ? gapR : 0);
                 const double xj = X[left ? jl : jr];
                 const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
@@ -588,14 +590,18 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
                 dS += in ? tt : 0.0;
                 terms += in ? 2u : 0u;
                 live = in && k < kS;
+                ++k;
+            } while (__any(live));
             }
             dE = dS + __shfl_xor(dS, 1);
             const double bf = exp(-dE * beta);
             return go && (dE <= 0.0 || bf > u);
         }
         float sS = 0.0f, cS = 0.0f;
+        uint32_t k = 1;
+        if (__any(live)) {
 #pragma unroll 1
-        for (uint32_t k = 1; __any(live); ++k) {
+        do {
             const uint32_t jl = my - k - (k > t ? gapL : 0), jr = my + k + (t + k >= size ? gapR : 0);
             const double xj = X[left ? jl : jr];
             const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
@@ -608,6 +614,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             sS = in ? uu : sS;
             terms += in ? 2u : 0u;
             live = in && k < kS;
+            ++k;
+        } while (__any(live));
         }
         const double mine = (double)sS - (double)cS;
         dE = mine + __shfl_xor(mine, 1);
@@ -617,8 +625,10 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     bool liveL = go && kL >= 1, liveR = go && kR >= 1;
     if (sizeof(Real) == 8) {
         double dEL = 0.0, dER = 0.0;
+        uint32_t k = 1;
+        if (__any(liveL || liveR)) {
 #pragma unroll 1
-        for (uint32_t k = 1; __any(liveL || liveR); ++k) {
+        do {
             const double xjL = X[my - k - (k > t ? gapL : 0)];
             const double xjR = X[my + k + (t + k >= size ? gapR : 0)];
             const double oL = xi - xjL, nL = xn - xjL;
@@ -632,14 +642,18 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
             liveL = inL && k < kL;
             liveR = inR && k < kR;
+            ++k;
+        } while (__any(liveL || liveR));
         }
         dE = dEL + dER;
         const double bf = exp(-dE * beta);
         return go && (dE <= 0.0 || bf > u);
     }
     float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
+    uint32_t k = 1;
+    if (__any(liveL || liveR)) {
 #pragma unroll 1
-    for (uint32_t k = 1; __any(liveL || liveR); ++k) {
+    do {
         const double xjL = X[my - k - (k > t ? gapL : 0)];
         const double xjR = X[my + k + (t + k >= size ? gapR : 0)];
         // separations are FP64 differences; the cutoff decisions are made on
@@ -666,6 +680,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
         liveL = inL && k < kL;
         liveR = inR && k < kR;
+        ++k;
+    } while (__any(liveL || liveR));
     }
     dE = ((double)sL - (double)cL) + ((double)sR - (double)cR);
     const float bf = fexp(-betaf * (float)dE);
@@ -704,8 +720,9 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     double xi = X[run ? a0 : 0u];
     double xis = xi * vp.sc;
     double x1 = 0.0, x1s = 0.0;
+    if (__any(run)) {
 #pragma unroll 1
-    while (__any(run)) {
+    do {
         const uint32_t i = start + t;
         const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
         const bool valid = run && k <= kR;
@@ -739,6 +756,7 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         xis = nextp ? x1s : xis;
         k = nextp ? 1u : k + 1;
         t = tn;
+    } while (__any(run));
     }
     return sizeof(Real) == 8 ? part : (double)sf - (double)cc;
 }
