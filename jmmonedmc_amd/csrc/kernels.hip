@@ -562,8 +562,10 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     const bool wall = !c.per && fabs(xn) > L / 2.0;
     const bool hasL = c.N >= 2 && (c.per || i > 0);
     const bool hasR = c.N >= 2 && (c.per || i + 1 < c.N);
-    const double xl = X[hasL ? my - 1 - (t == 0 ? gapL : 0) : my];
-    const double xr = X[hasR ? my + 1 + (t + 1 >= size ? gapR : 0) : my];
+    // the nearest slots on both sides (ghost or halo slots at a chain's
+    // ends: in the image; their values count only where hasL / hasR)
+    const double xl = X[my - 1 - (t == 0 ? gapL : 0)];
+    const double xr = X[my + 1 + (t + 1 >= size ? gapR : 0)];
     const bool core = !wall && ((hasL && (xn - xl) < gmin) || (hasR && (xr - xn) < gmin));
     n_wall += wall ? 1u : 0u;
     n_core += core ? 1u : 0u;
