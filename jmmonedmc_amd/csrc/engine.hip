@@ -507,6 +507,12 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         jmm_destroy(e);
         return rc;
     };
+    // from here on a failing HIP call releases the partly built engine
+#define CREATE_CHK(x)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return bail(fail(JMM_EHIP, "%s: %s", #x, hipGetErrorString(e_))); \
+    } while (0)
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(JMM_EHIP, "hipStreamCreate failed"));
 
@@ -517,9 +523,9 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         maxB = std::max(maxB, ch[c].B);
     }
     int lds_max = 0;
-    HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device));
+    CREATE_CHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device));
     int n_cu = 0;
-    HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device));
+    CREATE_CHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device));
 
     // chains whose LDS image does not fit one workgroup: split into S
     // segments of B/S blocks (S the smallest power of two whose segment image
@@ -627,9 +633,9 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         if (int rc = dalloc(&e->d_log, (size_t)e->cap * e->total_N)) return bail(rc);
     e->step_cap.resize(n_chains);
     for (int k = 0; k < n_chains; ++k) e->step_cap[k] = e->hch[k].max_step;
-    HIPCHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->d_states, hst.data(), sizeof(DevState) * n_chains, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
+    CREATE_CHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
+    CREATE_CHK(hipMemcpy(e->d_states, hst.data(), sizeof(DevState) * n_chains, hipMemcpyHostToDevice));
+    CREATE_CHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
     if (int rc = refresh_energies(e, -1)) return bail(rc);
     // launch groups: plain form by LDS fit; batched forms by (memory mode,
     // potential), one persistent launch each
@@ -653,11 +659,11 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             Group &Gr = e->grp.back();
             if (int rc = set_lds_attr(plain_kernel(g), Gr.lds)) return bail(rc);
             if (int rc = dalloc(&Gr.d_list, Gr.n_items)) return bail(rc);
-            HIPCHK(hipMemcpy(Gr.d_list, lists[g].data(), sizeof(uint32_t) * Gr.n_items, hipMemcpyHostToDevice));
+            CREATE_CHK(hipMemcpy(Gr.d_list, lists[g].data(), sizeof(uint32_t) * Gr.n_items, hipMemcpyHostToDevice));
         }
     } else {
         if (int rc = dalloc(&e->d_slots, slots.size())) return bail(rc);
-        HIPCHK(hipMemcpy(e->d_slots, slots.data(), sizeof(DevSlot) * slots.size(), hipMemcpyHostToDevice));
+        CREATE_CHK(hipMemcpy(e->d_slots, slots.data(), sizeof(DevSlot) * slots.size(), hipMemcpyHostToDevice));
         for (int g = 0; g < 2; ++g) {
             std::vector<DevBundle> bv = bundles[g];
             if (bv.empty()) continue;
@@ -674,12 +680,13 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             const void *fn = batched_kernel(e->form == JMM_FORM_BATCHED_FP32, e->T, g);
             if (int rc = set_lds_attr(fn, G.lds)) return bail(rc);
             int occ = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, e->T, G.lds));
+            CREATE_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, e->T, G.lds));
             if (occ < 1) occ = 1;
             G.grid = std::min<uint32_t>(G.n_items, (uint32_t)(occ * n_cu));
-            if (int rc = dalloc(&G.d_bundles, G.n_items)) return bail(rc);
-            HIPCHK(hipMemcpy(G.d_bundles, bv.data(), sizeof(DevBundle) * G.n_items, hipMemcpyHostToDevice));
-            e->grp.push_back(G);
+            e->grp.push_back(G);  // owned by the engine from here: destroy releases it
+            Group &Gb = e->grp.back();
+            if (int rc = dalloc(&Gb.d_bundles, Gb.n_items)) return bail(rc);
+            CREATE_CHK(hipMemcpy(Gb.d_bundles, bv.data(), sizeof(DevBundle) * Gb.n_items, hipMemcpyHostToDevice));
         }
     }
     // segmented chains: one cooperative launch, one workgroup per segment
@@ -718,21 +725,21 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             const void *fn = segmented_kernel(fp32, e->seg_tw, e->seg_lpb);
             if (int rc = set_lds_attr(fn, e->seg_lds_bytes)) return bail(rc);
             int occ = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, e->seg_tw, e->seg_lds_bytes));
+            CREATE_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, e->seg_tw, e->seg_lds_bytes));
             if ((uint64_t)occ * (uint64_t)n_cu < e->n_segs)
                 return bail(fail(JMM_EINVAL, "jmm_create: split chains need %u workgroups resident at once, "
                                              "the device holds %d", e->n_segs, occ * n_cu));
             if (int rc = dalloc(&e->d_segs, segv.size())) return bail(rc);
-            HIPCHK(hipMemcpy(e->d_segs, segv.data(), sizeof(DevSeg) * segv.size(), hipMemcpyHostToDevice));
+            CREATE_CHK(hipMemcpy(e->d_segs, segv.data(), sizeof(DevSeg) * segv.size(), hipMemcpyHostToDevice));
             if (int rc = dalloc(&e->d_pub_head, (size_t)e->n_segs * wgmax)) return bail(rc);
             if (int rc = dalloc(&e->d_pub_tail, (size_t)e->n_segs * wgmax)) return bail(rc);
             if (int rc = dalloc(&e->d_part, (size_t)e->n_segs * 2 * SEG_FIELDS * SEG_WAVES)) return bail(rc);
             if (int rc = dalloc(&e->d_ctr, e->n_segs)) return bail(rc);
             if (int rc = dalloc(&e->d_seg_err, 1)) return bail(rc);
             std::vector<unsigned long long> z(e->n_segs, 0ull);
-            HIPCHK(hipMemcpy(e->d_ctr, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
+            CREATE_CHK(hipMemcpy(e->d_ctr, z.data(), sizeof(unsigned long long) * z.size(), hipMemcpyHostToDevice));
             const uint32_t zero = 0;
-            HIPCHK(hipMemcpy(e->d_seg_err, &zero, sizeof zero, hipMemcpyHostToDevice));
+            CREATE_CHK(hipMemcpy(e->d_seg_err, &zero, sizeof zero, hipMemcpyHostToDevice));
             e->seg_ctrl = (uint32_t)e->grp.size();
             e->use_graph = 0;  // cooperative launches are not captured
         }
@@ -741,23 +748,24 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     if (int rc = dalloc(&e->d_ctrl, std::max<size_t>(1, e->n_ctrl))) return bail(rc);
     {
         std::vector<Ctrl> zc(std::max<size_t>(1, e->n_ctrl));
-        HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
+        CREATE_CHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
     }
 
     if (e->use_graph) {
-        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        CREATE_CHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         int rc = launch_block(e, e->block_sweeps);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        e->graph = g;  // released by destroy, also when the capture failed
         if (rc) return bail(rc);
         if (ce != hipSuccess) return bail(fail(JMM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ce)));
-        e->graph = g;
-        HIPCHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+        CREATE_CHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
     }
-    HIPCHK(hipStreamSynchronize(e->stream));
+    CREATE_CHK(hipStreamSynchronize(e->stream));
     *out = e;
     return 0;
 }
+#undef CREATE_CHK
 
 static int launch_hist_sample(jmm_engine *e)
 {
