@@ -150,3 +150,26 @@ def test_split_chain_replays_oracle(gpu, oracle, name, lanes):
     assert got["counters"]["sweeps"][0] == osw == SWEEPS
     assert got["counters"]["trials"][0] == orc.counter("trials")
     assert got["counters"]["acc_d"][0] == orc.counter("acc_d")
+
+
+def test_c5_chain_replays_oracle(gpu, oracle):
+    """A chain of the 65536-particle Yukawa workload, split because its LDS
+    image (0.53 MB) exceeds a workgroup's: 4 segments of 256 blocks, FP64
+    form, two launches, equal to the oracle move for move."""
+    from jmmonedmc_amd.configs import c5
+    ch = c5(1)[0]
+    print("[gpu] split oracle c5 chain N=65536", flush=True)
+    with Engine([ch], form="batched_fp64", sample_capacity=6, block_sweeps=3, log_accepts=True) as e:
+        lay = e.layout(0)
+        assert lay["mode"] == "split" and lay["segments"] == 4 and lay["B"] == 1024, lay
+        e.run(6)
+        log = e.read_accept_log(6)[0].ravel()
+        smp = e.read_samples(6)[0]
+        x, L, U, sw = e.state(0)
+    orc = oracle.Chain(oracle_spec(oracle, ch))
+    osmp, olog = orc.sweeps(6, log=True)
+    ox, oL, oU, osw = orc.state()
+    assert np.array_equal(log, olog)
+    assert np.array_equal(x, ox) and L == oL and sw == osw == 6
+    assert rel(U, oU) <= 1e-10
+    assert np.array_equal(smp["acc_v"], osmp["acc_v"])
