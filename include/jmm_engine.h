@@ -90,8 +90,9 @@ typedef struct jmm_engine_opts {
     uint32_t lds_budget;      /* batched forms: LDS bytes per workgroup (0 = 64 KiB) */
     uint32_t seg_lds;         /* batched forms: a chain whose LDS image exceeds this many bytes
                                  (0 = the device's LDS per workgroup) is split into segments of
-                                 consecutive blocks, one workgroup each, that synchronise per
-                                 phase; a chain that cannot be split runs from HBM */
+                                 consecutive blocks, one workgroup each, neighbours
+                                 synchronising every (block length - window) phases; a
+                                 chain that cannot be split runs from HBM */
     uint32_t seg_lanes;       /* split chains: lanes per block, 1 or 2 (two lanes share each
                                  trial, one per side; 0 = 1); results are the same bits
                                  either way */
