@@ -6,7 +6,9 @@ workload (C3).
 
 A step is one sweep block (--block-sweeps sweeps, default 100) of every chain;
 the default K=100 steps is the workload's full 1e4 sweeps.  Prints one JSON
-line.
+line.  Other workloads (--workload): c2, c5, and c4, whose steps also run
+the backward (U, W, dU/dtheta, dW/dtheta, dU/dx) of every chain after each
+block, timed with the sweeps and reported under "backward".
 """
 from __future__ import annotations
 
@@ -32,7 +34,7 @@ def parse():
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--block-sweeps", type=int, default=100)
-    p.add_argument("--workload", default="c3", choices=["c3", "c2", "c5"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"])
     p.add_argument("--plain-steps", type=int, default=1, help="sweep blocks timed for the plain FP64 form")
     p.add_argument("--forms", default="fp32,fp64,plain",
                    help="comma list of forms to time: fp32 (batched FP32, the value), fp64 "
@@ -55,6 +57,10 @@ def workload(name):
                               "+-0.01, step +-0.5, density 0.5 start, FP64 recompute every 100 sweeps")
     if name == "c2":
         return configs.c2(), "C2: 256 chains, N=4096, NPT T=1 P=0.5, shifted LJ rc=3"
+    if name == "c4":
+        return configs.c4(), ("C4: 1024 chains, N=1024, NVT T=1, rho=0.6, shifted LJ rc=3, "
+                              "(eps, sigma) = (1, 1); after every sweep block the backward "
+                              "(U, W, dU/dtheta, dW/dtheta, dU/dx) of every chain's configuration")
     return configs.c5(), "C5: 64 chains, N=65536, NPT T=1.5 P=1, Yukawa rc=20 + hard core 0.5"
 
 
@@ -78,7 +84,10 @@ def block_sem(x, nblocks=20):
     return float(m.std() / math.sqrt(nblocks))
 
 
-def run_form(chains, form, steps, warmup, bs, graph, label, opts=None):
+def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=False):
+    """Time `steps` sweep blocks; with `backward`, each block is followed by
+    the backward of every chain's configuration (jmm_observables with
+    dU/dx, results read back), inside the timed region."""
     from jmmonedmc_amd.engine import Engine
     cap = max(steps, warmup, 1) * bs
     o = dict(use_graph=bool(graph))
@@ -92,9 +101,15 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None):
     before = eng.all_counters()
     print(f"[bench] {label}: timing {steps} blocks of {bs} sweeps", file=sys.stderr, flush=True)
     eng.sync()
+    back_s = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.run(bs)
+        if backward:
+            eng.sync()
+            tb = time.perf_counter()
+            eng.observables(want_grad=True)
+            back_s += time.perf_counter() - tb
     eng.sync()
     t1 = time.perf_counter()
     launches, ms, sweeps = eng.kernel_time(reset=True)
@@ -104,9 +119,16 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None):
     readback = time.perf_counter() - t2
     eng.close()
     wall = t1 - t0
-    return dict(wall_s=wall, trials=trials, terms=terms, flops=flops, launches=launches,
-                kernel_ms=ms, sweeps=sweeps, samples=smp, readback_s=readback,
-                trials_per_s=trials / wall, ms_per_step=1e3 * wall / steps)
+    out = dict(wall_s=wall, trials=trials, terms=terms, flops=flops, launches=launches,
+               kernel_ms=ms, sweeps=sweeps, samples=smp, readback_s=readback,
+               trials_per_s=trials / wall, ms_per_step=1e3 * wall / steps)
+    if backward:
+        out["backward"] = {"configurations": steps * len(chains), "seconds": back_s,
+                           "configurations_per_s": steps * len(chains) / back_s if back_s else None,
+                           "ms_per_call": 1e3 * back_s / steps,
+                           "includes": "kernel and read-back of U, W, dU/dtheta, dW/dtheta and "
+                                       "dU/dx for every chain"}
+    return out
 
 
 def cpu_baseline(chains, sweeps, threads):
@@ -153,12 +175,13 @@ def main():
         print("[bench] autotuning launch parameters", file=sys.stderr, flush=True)
         tuned = tune(chains, form="batched_fp32", block_sweeps=a.block_sweeps)
         opts = dict(tuned.best)
+    bwd = a.workload == "c4"
     fp32 = run_form(chains, "batched_fp32", a.steps, a.warmup, a.block_sweeps, a.graph, "batched_fp32",
-                    opts)
+                    opts, backward=bwd)
     res = {}
     if "fp64" in forms:
         res["batched_fp64"] = run_form(chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps,
-                                       a.graph, "batched_fp64", opts)
+                                       a.graph, "batched_fp64", opts, backward=bwd)
     if "plain" in forms and a.plain_steps > 0:
         res["plain_fp64"] = run_form(chains, "plain_fp64", a.plain_steps, 0, a.block_sweeps, 0,
                                      "plain_fp64")
@@ -264,6 +287,8 @@ def main():
         "cpu_baseline": cpu,
         "accuracy_fp32_vs_fp64": accuracy,
     }
+    if "backward" in fp32:
+        out["backward"] = fp32["backward"]
     if "batched_fp64" in res:
         b = res["batched_fp64"]
         out["batched_fp64"] = {"value": b["trials_per_s"],

@@ -193,7 +193,10 @@ int jmm_full_energy(jmm_engine *e, double *U_out);
 /* Replaces updateThermo/printThermo's data path (src/jmmMCState.cpp:
  * 1896-1961): copies every chain's samples since the last read into
  * out[c * max_sweeps + s] and empties the device buffer.  *n_out = sweeps
- * copied per chain (all chains have the same count).  Synchronous. */
+ * copied per chain (all chains have the same count).  The samples are put
+ * in this layout on the device (the first read allocates a staging buffer of
+ * sample_capacity x n_chains x 24 bytes, kept until destroy) and arrive in
+ * one copy.  Synchronous. */
 int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out);
 
 /* Replaces printCoords (src/jmmMCState.cpp:1007-1017) and the restart read

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -231,6 +232,8 @@ struct jmm_engine {
     Ctrl *d_ctrl = nullptr;
     double *d_sU = nullptr, *d_sL = nullptr;
     uint2 *d_sA = nullptr;
+    SampleRec *d_packed = nullptr;  // samples in the caller's layout (first read allocates)
+    uint32_t *d_ext = nullptr;      // external index of each internal chain
     uint8_t *d_log = nullptr;
     double *d_obs = nullptr, *d_grad = nullptr, *d_Ltmp = nullptr, *d_Lnew = nullptr;
     uint32_t *d_all = nullptr;
@@ -463,6 +466,8 @@ int jmm_destroy(jmm_engine *e)
     (void)hipFree(e->d_all);
     (void)hipFree(e->d_rho);
     (void)hipFree(e->d_g);
+    (void)hipFree(e->d_packed);
+    (void)hipFree(e->d_ext);
     (void)hipFree(e->d_segs);
     (void)hipFree(e->d_pub_head);
     (void)hipFree(e->d_pub_tail);
@@ -920,6 +925,11 @@ int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *swe
     return 0;
 }
 
+static_assert(sizeof(jmm_sample) == sizeof(SampleRec) && offsetof(jmm_sample, L) == offsetof(SampleRec, L) &&
+                  offsetof(jmm_sample, acc_d) == offsetof(SampleRec, acc_d) &&
+                  offsetof(jmm_sample, acc_v) == offsetof(SampleRec, acc_v),
+              "the device sample record must have the ABI's layout");
+
 int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out)
 {
     if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_samples: null argument");
@@ -929,23 +939,20 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
     const uint32_t n = e->write_pos;
     if (n > max_sweeps) return fail(JMM_EINVAL, "jmm_read_samples: %u sweeps pending, room for %u", n, max_sweeps);
     if (n && !out) return fail(JMM_EINVAL, "jmm_read_samples: out is NULL");
-    const size_t rows = (size_t)n * e->n;
-    std::vector<double> U(rows), L(rows);
-    std::vector<uint2> A(rows);
-    if (rows) {
-        HIPCHK(hipMemcpy(U.data(), e->d_sU, rows * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(L.data(), e->d_sL, rows * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(A.data(), e->d_sA, rows * sizeof(uint2), hipMemcpyDeviceToHost));
-    }
-    for (uint32_t s = 0; s < n; ++s)
-        for (int k = 0; k < e->n; ++k) {
-            const size_t r = (size_t)s * e->n + k;
-            jmm_sample &o = out[(size_t)e->ext_of_int[k] * max_sweeps + s];
-            o.U = U[r];
-            o.L = L[r];
-            o.acc_d = A[r].x;
-            o.acc_v = A[r].y;
+    if (n) {
+        // transposed to [chain][sweep] on the device, then one pitched copy
+        // into the caller's rows of max_sweeps samples
+        if (!e->d_packed) {
+            if (int rc = dalloc(&e->d_packed, (size_t)e->cap * e->n)) return rc;
+            if (int rc = dalloc(&e->d_ext, (size_t)e->n)) return rc;
+            HIPCHK(hipMemcpy(e->d_ext, e->ext_of_int.data(), sizeof(uint32_t) * e->n, hipMemcpyHostToDevice));
         }
+        HIPCHK(launch_pack_samples(e->d_sU, e->d_sL, e->d_sA, e->d_ext, (uint32_t)e->n, n, e->d_packed,
+                                   e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy2D(out, (size_t)max_sweeps * sizeof(jmm_sample), e->d_packed, (size_t)n * sizeof(SampleRec),
+                           (size_t)n * sizeof(SampleRec), (size_t)e->n, hipMemcpyDeviceToHost));
+    }
     e->write_pos = 0;
     std::vector<Ctrl> zc(std::max<size_t>(1, e->n_ctrl));
     HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));

@@ -138,6 +138,15 @@ const void *plain_kernel(int gx);
 hipError_t launch_plain(const SweepArgs &a, int gx, uint32_t grid, size_t lds, hipStream_t s);
 hipError_t launch_observables(const ObsArgs &a, int fp32, uint32_t grid, hipStream_t s);
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s);
+// Samples in the caller's layout (jmm_sample): out[ext[k] * n_sweeps + s]
+// from the engine's [sweep][chain] arrays.
+struct SampleRec {
+    double U, L;
+    uint32_t acc_d, acc_v;
+};
+static_assert(sizeof(SampleRec) == 24, "SampleRec must match jmm_sample");
+hipError_t launch_pack_samples(const double *sU, const double *sL, const uint2 *sA, const uint32_t *ext,
+                               uint32_t n_chains, uint32_t n_sweeps, SampleRec *out, hipStream_t s);
 hipError_t launch_hist(const HistArgs &a, hipStream_t s);
 // tw: workgroup width (256 or 512); lpb: lanes per block (1 or 2; 512 only with 2)
 const void *segmented_kernel(int fp32, int tw, int lpb);

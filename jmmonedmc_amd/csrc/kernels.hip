@@ -1717,6 +1717,46 @@ __global__ void k_lengths(const DevState *st, double *L, uint32_t n)
     if (i < n) L[i] = st[i].L;
 }
 
+// Sample rows [sweep][chain] (internal chain order) to the caller's layout
+// [chain][sweep] (external order), through 32 x 32 tiles in LDS so that the
+// reads (along chains) and the writes (along sweeps) are both contiguous.
+// Grid-stride over tiles.
+__global__ void __launch_bounds__(256) k_pack_samples(const double *sU, const double *sL, const uint2 *sA,
+                                                      const uint32_t *ext, uint32_t nc, uint32_t ns,
+                                                      SampleRec *out)
+{
+    __shared__ double tU[32][33], tL[32][33];
+    __shared__ uint2 tA[32][33];
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;
+    const uint32_t tk = (nc + 31u) / 32u, ts = (ns + 31u) / 32u;
+    const uint64_t tiles = (uint64_t)tk * ts;
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t k0 = (uint32_t)(tile % tk) * 32u, s0 = (uint32_t)(tile / tk) * 32u;
+        for (uint32_t j = ty; j < 32u; j += 8u) {
+            const uint32_t sw = s0 + j, k = k0 + tx;
+            if (sw < ns && k < nc) {
+                const uint64_t r = (uint64_t)sw * nc + k;
+                tU[j][tx] = sU[r];
+                tL[j][tx] = sL[r];
+                tA[j][tx] = sA[r];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = ty; j < 32u; j += 8u) {
+            const uint32_t k = k0 + j, sw = s0 + tx;
+            if (k < nc && sw < ns) {
+                SampleRec o;
+                o.U = tU[tx][j];
+                o.L = tL[tx][j];
+                o.acc_d = tA[tx][j].x;
+                o.acc_v = tA[tx][j].y;
+                out[(uint64_t)ext[k] * ns + sw] = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ histograms
 
 // Position folded into [-L/2, L/2) on a periodic line; unchanged with walls.
@@ -1869,6 +1909,16 @@ hipError_t launch_hist(const HistArgs &a, hipStream_t s)
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_lengths, dim3((n + 255) / 256), dim3(256), 0, s, st, L, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_samples(const double *sU, const double *sL, const uint2 *sA, const uint32_t *ext,
+                               uint32_t n_chains, uint32_t n_sweeps, SampleRec *out, hipStream_t s)
+{
+    const uint64_t tiles = (uint64_t)((n_chains + 31u) / 32u) * ((n_sweeps + 31u) / 32u);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, 1u << 16);
+    if (!grid) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_samples, dim3(grid), dim3(256), 0, s, sU, sL, sA, ext, n_chains, n_sweeps, out);
     return hipGetLastError();
 }
 

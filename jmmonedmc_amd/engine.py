@@ -13,6 +13,10 @@ import numpy as np
 from . import _native as nat
 from ._native import ChainSpec, Counters, EngineOpts, HistSpec, Sample, check, lib
 
+# numpy view of jmm_sample
+SAMPLE_DTYPE = np.dtype([("U", "f8"), ("L", "f8"), ("acc_d", "u4"), ("acc_v", "u4")])
+assert SAMPLE_DTYPE.itemsize == C.sizeof(Sample)
+
 
 @dataclass
 class Chain:
@@ -116,13 +120,14 @@ class Engine:
         check(lib().jmm_synchronize(self.h), "jmm_synchronize")
 
     def read_samples(self, max_sweeps: int | None = None):
-        """Structured array [chain, sweep] with fields U, L, acc_d, acc_v."""
+        """Structured array [chain, sweep] with fields U, L, acc_d, acc_v
+        (the library writes the rows straight into it)."""
         m = max_sweeps or self.cap
-        buf = (Sample * (self.n * m))()
+        out = np.empty((self.n, m), dtype=SAMPLE_DTYPE)
         n = C.c_uint32()
-        check(lib().jmm_read_samples(self.h, buf, m, C.byref(n)), "jmm_read_samples")
-        raw = np.frombuffer(buf, dtype=[("U", "f8"), ("L", "f8"), ("acc_d", "u4"), ("acc_v", "u4")])
-        return raw.reshape(self.n, m)[:, : n.value].copy()
+        check(lib().jmm_read_samples(self.h, out.ctypes.data_as(C.POINTER(Sample)), m, C.byref(n)),
+              "jmm_read_samples")
+        return out if n.value == m else out[:, : n.value].copy()
 
     def read_accept_log(self, max_sweeps: int | None = None):
         """List (per chain) of uint8 arrays [sweep, particle]; read before read_samples."""
