@@ -234,6 +234,7 @@ struct jmm_engine {
     uint2 *d_sA = nullptr;
     SampleRec *d_packed = nullptr;  // samples in the caller's layout (first read allocates)
     uint32_t *d_ext = nullptr;      // external index of each internal chain
+    uint64_t *d_extoff = nullptr;   // external position offset of each internal chain
     uint8_t *d_log = nullptr;
     double *d_obs = nullptr, *d_grad = nullptr, *d_Ltmp = nullptr, *d_Lnew = nullptr;
     uint32_t *d_all = nullptr;
@@ -468,6 +469,7 @@ int jmm_destroy(jmm_engine *e)
     (void)hipFree(e->d_g);
     (void)hipFree(e->d_packed);
     (void)hipFree(e->d_ext);
+    (void)hipFree(e->d_extoff);
     (void)hipFree(e->d_segs);
     (void)hipFree(e->d_pub_head);
     (void)hipFree(e->d_pub_tail);
@@ -925,6 +927,20 @@ int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *swe
     return 0;
 }
 
+// Device copies of the internal -> external chain maps (index and position
+// offset), for kernels that write results in the caller's order.
+static int ensure_ext(jmm_engine *e)
+{
+    if (e->d_ext) return 0;
+    std::vector<uint64_t> off(e->n);
+    for (int k = 0; k < e->n; ++k) off[k] = e->ext_off[e->ext_of_int[k]];
+    if (int rc = dalloc(&e->d_ext, (size_t)e->n)) return rc;
+    if (int rc = dalloc(&e->d_extoff, (size_t)e->n)) return rc;
+    HIPCHK(hipMemcpy(e->d_ext, e->ext_of_int.data(), sizeof(uint32_t) * e->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_extoff, off.data(), sizeof(uint64_t) * e->n, hipMemcpyHostToDevice));
+    return 0;
+}
+
 static_assert(sizeof(jmm_sample) == sizeof(SampleRec) && offsetof(jmm_sample, L) == offsetof(SampleRec, L) &&
                   offsetof(jmm_sample, acc_d) == offsetof(SampleRec, acc_d) &&
                   offsetof(jmm_sample, acc_v) == offsetof(SampleRec, acc_v),
@@ -942,11 +958,9 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
     if (n) {
         // transposed to [chain][sweep] on the device, then one pitched copy
         // into the caller's rows of max_sweeps samples
-        if (!e->d_packed) {
+        if (int rc = ensure_ext(e)) return rc;
+        if (!e->d_packed)
             if (int rc = dalloc(&e->d_packed, (size_t)e->cap * e->n)) return rc;
-            if (int rc = dalloc(&e->d_ext, (size_t)e->n)) return rc;
-            HIPCHK(hipMemcpy(e->d_ext, e->ext_of_int.data(), sizeof(uint32_t) * e->n, hipMemcpyHostToDevice));
-        }
         HIPCHK(launch_pack_samples(e->d_sU, e->d_sL, e->d_sA, e->d_ext, (uint32_t)e->n, n, e->d_packed,
                                    e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
@@ -1134,6 +1148,7 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32,
                             const double *Lnew_dev = nullptr)
 {
     HIPCHK(hipSetDevice(e->device));
+    if (int rc = ensure_ext(e)) return rc;
     if (!e->d_obs)
         if (int rc = dalloc(&e->d_obs, (size_t)e->n * 6)) return rc;
     if (!e->d_Ltmp)
@@ -1155,22 +1170,15 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32,
     a.Lnew = Lnew_dev;
     a.obs = e->d_obs;
     a.grad = grad ? e->d_grad : nullptr;
+    // results in the caller's chain order, copied out as they are
+    a.row = e->d_ext;
+    a.goff = e->d_extoff;
     a.n = (uint32_t)e->n;
     a.standalone = 0;
     HIPCHK(launch_observables(a, fp32, (uint32_t)e->n, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    std::vector<double> ho((size_t)e->n * 6);
-    HIPCHK(hipMemcpy(ho.data(), e->d_obs, sizeof(double) * ho.size(), hipMemcpyDeviceToHost));
-    for (int k = 0; k < e->n; ++k)
-        for (int m = 0; m < 6; ++m) obs[(size_t)e->ext_of_int[k] * 6 + m] = ho[(size_t)k * 6 + m];
-    if (grad) {
-        std::vector<double> hg(e->total_N);
-        HIPCHK(hipMemcpy(hg.data(), e->d_grad, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));
-        for (int k = 0; k < e->n; ++k) {
-            const uint32_t c = e->ext_of_int[k];
-            std::memcpy(grad + e->ext_off[c], hg.data() + e->hch[k].x_off, sizeof(double) * e->hch[k].N);
-        }
-    }
+    HIPCHK(hipMemcpy(obs, e->d_obs, sizeof(double) * (size_t)e->n * 6, hipMemcpyDeviceToHost));
+    if (grad) HIPCHK(hipMemcpy(grad, e->d_grad, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));
     return 0;
 }
 

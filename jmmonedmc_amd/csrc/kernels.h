@@ -87,6 +87,8 @@ struct ObsArgs {
     const double *Lnew;      // engine form, nullable: evaluate at x * Lnew/L, length Lnew
     double *obs;
     double *grad;
+    const uint32_t *row;     // engine form, nullable: output row of each chain (else its index)
+    const uint64_t *goff;    // engine form, nullable: gradient offset of each chain (else x_off)
     uint32_t n;
     uint32_t standalone;
 };

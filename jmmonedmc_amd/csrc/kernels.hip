@@ -1698,8 +1698,8 @@ __global__ void __launch_bounds__(256) k_observables(ObsArgs a)
             sc = a.Lnew[ic] / L;
             L = a.Lnew[ic];
         }
-        grad = a.grad ? a.grad + ch.x_off : nullptr;
-        obs = a.obs + (uint64_t)ic * 6;
+        grad = a.grad ? a.grad + (a.goff ? a.goff[ic] : ch.x_off) : nullptr;
+        obs = a.obs + (uint64_t)(a.row ? a.row[ic] : ic) * 6;
     }
     switch (ch.kind) {
     case POT_LJ: run_obs<Real, POT_LJ>(ch, X, L, sc, obs, grad); break;

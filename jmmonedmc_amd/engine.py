@@ -209,8 +209,8 @@ class Engine:
         return U
 
     def observables(self, want_grad: bool = False):
-        obs = np.zeros(self.n * 6)
-        g = np.zeros(int(self.Ns.sum())) if want_grad else None
+        obs = np.empty(self.n * 6)
+        g = np.empty(int(self.Ns.sum())) if want_grad else None
         check(lib().jmm_observables(self.h, _dp(obs), _dp(g) if want_grad else None),
               "jmm_observables")
         obs = obs.reshape(self.n, 6)
