@@ -173,3 +173,34 @@ def test_c5_chain_replays_oracle(gpu, oracle):
     assert np.array_equal(x, ox) and L == oL and sw == osw == 6
     assert rel(U, oU) <= 1e-10
     assert np.array_equal(smp["acc_v"], osmp["acc_v"])
+
+
+def test_every_placement_in_one_engine_replays_oracle(gpu, oracle):
+    """One engine holding a chain that runs from HBM (24000 particles with
+    one block: too large for a workgroup and nothing to split), a chain in
+    LDS and a split chain, FP64 form: each equal to the oracle move for
+    move (the three launch groups run one after another on one stream)."""
+    hbm = Chain(N=24000, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT, vol_move=nat.VOL_LOG,
+                max_dv=0.01, seed=31, chain_id=31, bcap=1, **lj())
+    whole = Chain(N=64, L0=128.0, T=1.0, seed=32, chain_id=32, **lj())
+    split = split_case("lj_periodic_npt_log")
+    seg_lds, _ = seg_lds_for(oracle, split, 16)
+    chains = [hbm, whole, split]
+    print("[gpu] placements: hbm + lds + split", flush=True)
+    with Engine(chains, form="batched_fp64", sample_capacity=4, block_sweeps=2, seg_lds=seg_lds,
+                log_accepts=True) as e:
+        modes = [e.layout(c)["mode"] for c in range(3)]
+        assert modes == ["hbm", "lds", "split"], modes
+        e.run(4)
+        logs = e.read_accept_log(4)
+        smp = e.read_samples(4)
+        states = [e.state(c) for c in range(3)]
+    for c, ch in enumerate(chains):
+        orc = oracle.Chain(oracle_spec(oracle, ch))
+        osmp, olog = orc.sweeps(4, log=True)
+        ox, oL, oU, osw = orc.state()
+        x, L, U, sw = states[c]
+        assert np.array_equal(logs[c].ravel(), olog), c
+        assert np.array_equal(x, ox) and L == oL and sw == osw == 4, c
+        assert rel(U, oU) <= 1e-10, c
+        assert np.array_equal(smp[c]["acc_v"], osmp["acc_v"]), c
