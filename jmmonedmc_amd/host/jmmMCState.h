@@ -1,9 +1,11 @@
 // jmmMCState.h -- the reference's opaque-state chain API
 // (src/jmmMCState.h:6-48) on top of the engine's C ABI (include/jmm_engine.h).
 //
-// Same lifecycle and names: setupMCS -> incrementStep/Step/print* -> freeMCS,
-// one host thread (the reference's functions must be called by every OpenMP
-// thread; these need one caller).  What a "step" is differs: Step() advances
+// Same lifecycle and names: setupMCS -> incrementStep/Step/print* -> freeMCS.
+// The reference's collective functions (fad, incrementStep, Step, printG,
+// relaxVolume) may be called by every thread of an OpenMP team, as
+// src/Main.cpp does, or from one thread: one thread does the work and every
+// caller gets its result.  What a "step" is differs: Step() advances
 // the chain by one sweep (N displacement trials in the engine's block order,
 // then one box-length trial for NPT), i.e. numTrialTypes reference trials, and
 // the step counter counts trials as in the reference.  Interval predicates
