@@ -159,7 +159,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this "
-                                   "build (chains shard across GPUs by design, see DESIGN.md)",
+                                   "build (chains are independent and would shard across GPUs, one engine per GPU)",
                           "n_gpus": a.gpus}))
         return 2
     if a.calibrate_copy:
