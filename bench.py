@@ -131,14 +131,34 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
     return out
 
 
-def cpu_baseline(chains, sweeps, threads):
+def cpu_baseline(chains, sweeps, threads, allpairs=False):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.build()
     specs = [pyoracle.make_spec(**ch.as_dict()) for ch in chains]
-    dt, _ = pyoracle.run_batch(specs, sweeps, threads)
+    dt, _ = pyoracle.run_batch(specs, sweeps, threads, allpairs=allpairs)
     trials = sweeps * sum(ch.N for ch in chains)
     return trials / dt, dt, trials
+
+
+def cpu_extra(chains, n_trials_per_sweep):
+    """The oracle on one core over the same workload, and the reference's
+    algorithm (every other particle a partner of every trial, as qad2,
+    src/jmmMCState.cpp:1213-1348) on one core for C1 and one C2 chain."""
+    from jmmonedmc_amd import configs
+    sw1 = max(1, int(8.0 * 1.2e7 / n_trials_per_sweep))
+    v1, dt1, tr1 = cpu_baseline(chains, sw1, 1)
+    out = {"single_core": {"value": v1, "unit": "trial moves/s", "cores": 1,
+                           "sample": f"FP64 oracle, all {len(chains)} chains x {sw1} sweeps "
+                                     f"({tr1:.3g} trials, {dt1:.1f} s)"}}
+    ref = {"unit": "trial moves/s", "cores": 1,
+           "note": "FP64 oracle with every other particle a partner of every trial "
+                   "(the reference's O(N) loop), not the cutoff window"}
+    for name, chs, sw in (("c1", configs.c1(), 10000), ("c2_one_chain", configs.c2(n_chains=1), 100)):
+        v, dt, tr = cpu_baseline(chs, sw, 1, allpairs=True)
+        ref[name] = {"value": v, "sample": f"N={chs[0].N}, {sw} sweeps ({tr:.3g} trials, {dt:.1f} s)"}
+    out["reference_algorithm"] = ref
+    return out
 
 
 def calibrate_copy(mib):
@@ -256,6 +276,7 @@ def main():
         cpu = {"value": v, "unit": "trial moves/s", "cores": threads, "kind": "port",
                "sample": f"FP64 oracle, all {len(chains)} chains x {sw} sweeps ({tr:.3g} trials, "
                          f"{dt:.1f} s), one chain per thread"}
+        cpu.update(cpu_extra(chains, n_trials_per_sweep))
 
     plain = res.get("plain_fp64")
     out = {
