@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "trial moves/s on 1 MI355X, batched FP32 vs plain FP64, at ≤3σ ⟨U⟩ agreement"
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+# v_exp_f32 / v_rcp_f32 / v_sqrt_f32 issue at a quarter of the FP32 lane rate:
+# 256 CUs x 64 lanes / 4 per clock x 2.4 GHz
+PEAK_TRANS_TOPS = 256 * 64 / 4 * 2.4e9 / 1e12
 
 
 def parse():
@@ -66,14 +69,18 @@ def workload(name):
 
 def algorithmic_flops(eng, chains, before):
     """sum over chains of f_pair*(n_old+n_new) + (n_old+n_new) + 8 per trial,
-    with n_old+n_new the pair evaluations the kernel counted."""
-    from jmmonedmc_amd.configs import flop_per_pair
+    with n_old+n_new the pair evaluations the kernel counted; and the
+    transcendental instructions those need (per pair by potential, plus the
+    Metropolis exponential of every trial)."""
+    from jmmonedmc_amd.configs import flop_per_pair, trans_per_pair
     cnt = eng.all_counters()
     terms = (cnt["pair_terms"] - before["pair_terms"]).astype(np.float64)
     trials = (cnt["trials"] - before["trials"]).astype(np.float64)
     fp = np.array([flop_per_pair(ch.potential) for ch in chains], dtype=np.float64)
     tot = float((fp * terms + terms + 8.0 * trials).sum())
-    return tot, int(terms.sum()), int(trials.sum())
+    tp = np.array([trans_per_pair(ch.potential) for ch in chains], dtype=np.float64)
+    trans = float((tp * terms + trials).sum())
+    return tot, int(terms.sum()), int(trials.sum()), trans
 
 
 def block_sem(x, nblocks=20):
@@ -113,13 +120,13 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
     eng.sync()
     t1 = time.perf_counter()
     launches, ms, sweeps = eng.kernel_time(reset=True)
-    flops, terms, trials = algorithmic_flops(eng, chains, before)
+    flops, terms, trials, trans = algorithmic_flops(eng, chains, before)
     t2 = time.perf_counter()
     smp = eng.read_samples(cap)
     readback = time.perf_counter() - t2
     eng.close()
     wall = t1 - t0
-    out = dict(wall_s=wall, trials=trials, terms=terms, flops=flops, launches=launches,
+    out = dict(wall_s=wall, trials=trials, terms=terms, flops=flops, trans=trans, launches=launches,
                kernel_ms=ms, sweeps=sweeps, samples=smp, readback_s=readback,
                trials_per_s=trials / wall, ms_per_step=1e3 * wall / steps)
     if backward:
@@ -262,6 +269,10 @@ def main():
             "flops_per_launch": fp32["flops"] / max(1, fp32["launches"]),
             "avg_launch_ms": fp32["kernel_ms"] / max(1, fp32["launches"]),
             "pair_evals_per_trial": fp32["terms"] / max(1, fp32["trials"])}
+    trans_achieved = fp32["trans"] / (fp32["kernel_ms"] * 1e-3) / 1e12
+    roof["secondary"] = {"bound": "transcendental", "achieved": trans_achieved, "peak": PEAK_TRANS_TOPS,
+                         "unit": "Top/s", "frac": trans_achieved / PEAK_TRANS_TOPS,
+                         "per_trial": fp32["trans"] / max(1, fp32["trials"])}
     traffic_file = os.path.join(ROOT, "profiles", f"{a.workload}_sweep_traffic.txt")
     if os.path.exists(traffic_file):
         for line in open(traffic_file):

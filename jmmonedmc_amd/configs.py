@@ -74,3 +74,10 @@ def flop_per_pair(potential: int) -> float:
     """Algorithmic FLOPs per pair evaluation (add/mul/compare = 1, FMA = 2,
     rcp/div/sqrt/exp = 1): LJ 10, hard rod + well 4, Yukawa 10, harmonic 4."""
     return {nat.POT_LJ: 10.0, nat.POT_HRSW: 4.0, nat.POT_YUKAWA: 10.0, nat.POT_HARMONIC: 4.0}[potential]
+
+
+def trans_per_pair(potential: int) -> float:
+    """Quarter-rate transcendental instructions per FP32 pair evaluation:
+    LJ one reciprocal, Yukawa an exponential and a reciprocal, none for the
+    hard rod + well and the harmonic spring."""
+    return {nat.POT_LJ: 1.0, nat.POT_HRSW: 0.0, nat.POT_YUKAWA: 2.0, nat.POT_HARMONIC: 0.0}[potential]
