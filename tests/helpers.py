@@ -66,6 +66,24 @@ CASES = {
     # Yukawa + hard core, NPT periodic (C5 shape, small)
     "yukawa_npt": lambda: [Chain(N=200, rho=0.8, T=1.5, P=1.0, ensemble=nat.NPT, max_dv=0.01,
                                  seed=3, **yukawa())],
+    # the smallest chains, one bundle: a lone particle (no partners) with
+    # walls and periodic, pairs whose two images lie one on each side of a
+    # ring, w = N - 1 windows, every potential
+    "tiny_chains": lambda: [
+        Chain(N=1, L0=4.0, T=1.0, P=0.5, ensemble=nat.NPT, boundary=nat.WALLS,
+              vol_move=nat.VOL_LINEAR, max_dv=0.5, seed=21, chain_id=21, **lj(shifted=0)),
+        Chain(N=1, L0=10.0, T=1.0, seed=22, chain_id=22, **lj()),
+        Chain(N=2, L0=3.0, T=0.7, boundary=nat.WALLS, max_step=0.3, seed=23, chain_id=23, **hrsw()),
+        Chain(N=2, L0=9.0, T=1.0, P=0.3, ensemble=nat.NPT, vol_move=nat.VOL_LOG, max_dv=0.05,
+              seed=24, chain_id=24, **lj()),
+        Chain(N=3, L0=60.0, T=1.5, seed=25, chain_id=25, **yukawa()),
+        Chain(N=5, L0=6.0, T=0.4, P=0.7, ensemble=nat.NPT, boundary=nat.WALLS,
+              vol_move=nat.VOL_LINEAR, max_dv=0.5, max_step=0.1, seed=26, chain_id=26,
+              potential=nat.POT_HARMONIC, cutoff=math.inf, pot_p=(0, 0, 0, 0), shifted=0),
+        Chain(N=5, L0=8.0, T=0.9, P=1.0, ensemble=nat.NPT, boundary=nat.WALLS,
+              vol_move=nat.VOL_LINEAR, max_dv=0.1, max_step=0.1, seed=27, chain_id=27,
+              **lj(cutoff=math.inf, shifted=0)),
+    ],
     # a mixed batch: several chains per workgroup, several bundles, recompute
     "mixed_batch": lambda: (
         [Chain(N=n, rho=0.5, T=0.5 + 0.1 * k, P=0.2 + 0.1 * k, ensemble=nat.NPT, seed=k,
