@@ -548,7 +548,12 @@ __device__ __forceinline__ bool walk_in(const Pot &q, double o, double n, bool i
 // ends), then the two sums are exchanged and added.  Addition commutes, so
 // both lanes hold the one-lane result bit for bit.  Both lanes of a pair
 // must be active together.
-template <typename Real, int KIND, bool PAIR = false>
+//
+// W2 (FP32, one lane): two partners per side per loop iteration, four
+// independent pair evaluations in flight instead of two and half the
+// loop-exit tests; the terms are added in the same order with the same
+// freeze rule, so the result is the one-step walk's bit for bit.
+template <typename Real, int KIND, bool PAIR = false, bool W2 = false>
 __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, double gmin,
                                           double beta, float betaf, uint32_t i, uint32_t t,
                                           uint32_t size, uint32_t my, uint32_t gapL,
@@ -653,6 +658,62 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     }
     float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
     uint32_t k = 1;
+    if (W2) {
+    if (__any(liveL || liveR)) {
+#pragma unroll 1
+    do {
+        // partners k and k + 1 on both sides: a side still walking after
+        // step k has k < its limit <= w, so partner k + 1 is in the image;
+        // the clamp keeps the other lanes' reads there too
+        const uint32_t k2 = min(k + 1, c.w);
+        const double xjL = X[my - k - (k > t ? gapL : 0)];
+        const double xjR = X[my + k + (t + k >= size ? gapR : 0)];
+        const double xjL2 = X[my - k2 - (k2 > t ? gapL : 0)];
+        const double xjR2 = X[my + k2 + (t + k2 >= size ? gapR : 0)];
+        const double oL = xi - xjL, nL = xn - xjL;
+        const double oR = xjR - xi, nR = xjR - xn;
+        const bool ioL = inside64<KIND>(q, oL), inL64 = inside64<KIND>(q, nL);
+        const bool ioR = inside64<KIND>(q, oR), inR64 = inside64<KIND>(q, nR);
+        const bool inL = liveL && walk_in<KIND>(q, oL, nL, ioL, inL64);
+        const bool inR = liveR && walk_in<KIND>(q, oR, nR, ioR, inR64);
+        const float tL = phi32<KIND>(q, (float)nL, inL64) - phi32<KIND>(q, (float)oL, ioL);
+        const float tR = phi32<KIND>(q, (float)nR, inR64) - phi32<KIND>(q, (float)oR, ioR);
+        const bool liveL2 = inL && k < kL, liveR2 = inR && k < kR;
+        const double oL2 = xi - xjL2, nL2 = xn - xjL2;
+        const double oR2 = xjR2 - xi, nR2 = xjR2 - xn;
+        const bool ioL2 = inside64<KIND>(q, oL2), inL642 = inside64<KIND>(q, nL2);
+        const bool ioR2 = inside64<KIND>(q, oR2), inR642 = inside64<KIND>(q, nR2);
+        const bool inL2 = liveL2 && walk_in<KIND>(q, oL2, nL2, ioL2, inL642);
+        const bool inR2 = liveR2 && walk_in<KIND>(q, oR2, nR2, ioR2, inR642);
+        const float tL2 = phi32<KIND>(q, (float)nL2, inL642) - phi32<KIND>(q, (float)oL2, ioL2);
+        const float tR2 = phi32<KIND>(q, (float)nR2, inR642) - phi32<KIND>(q, (float)oR2, ioR2);
+        {
+            const float yL = tL - cL;
+            const float uL = sL + yL;
+            cL = inL ? (uL - sL) - yL : cL;
+            sL = inL ? uL : sL;
+            const float yR = tR - cR;
+            const float uR = sR + yR;
+            cR = inR ? (uR - sR) - yR : cR;
+            sR = inR ? uR : sR;
+        }
+        {
+            const float yL = tL2 - cL;
+            const float uL = sL + yL;
+            cL = inL2 ? (uL - sL) - yL : cL;
+            sL = inL2 ? uL : sL;
+            const float yR = tR2 - cR;
+            const float uR = sR + yR;
+            cR = inR2 ? (uR - sR) - yR : cR;
+            sR = inR2 ? uR : sR;
+        }
+        terms += (inL ? 2u : 0u) + (inR ? 2u : 0u) + (inL2 ? 2u : 0u) + (inR2 ? 2u : 0u);
+        liveL = inL2 && k + 1 < kL;
+        liveR = inR2 && k + 1 < kR;
+        k += 2;
+    } while (__any(liveL || liveR));
+    }
+    } else {
     if (__any(liveL || liveR)) {
 #pragma unroll 1
     do {
@@ -684,6 +745,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         liveR = inR && k < kR;
         ++k;
     } while (__any(liveL || liveR));
+    }
     }
     dE = ((double)sL - (double)cL) + ((double)sR - (double)cR);
     const float bf = fexp(-betaf * (float)dE);
@@ -858,9 +920,9 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
                 const double xn = xi + md;
                 ++n_tr;
                 double dE = 0.0;
-                const bool accd = trial_lds<Real, KIND>(c, q, gmin, beta, betaf, i, t, size, my,
-                                                        gapL, gapR, L, xi, xn, md, u, dE, n_terms,
-                                                        n_wall, n_core);
+                const bool accd = trial_lds<Real, KIND, false, true>(c, q, gmin, beta, betaf, i, t, size,
+                                                                     my, gapL, gapR, L, xi, xn, md, u, dE,
+                                                                     n_terms, n_wall, n_core);
                 const double xs = accd ? xn : xi;
                 c.R[my] = xs;
                 if (c.per) {
