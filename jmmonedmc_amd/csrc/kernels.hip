@@ -769,8 +769,11 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 // scaled position are kept rather than read again.  `last`: the lane holds
 // the chain's last block of a periodic chain, whose partners past the end
 // are the first particles shifted by L; wrap(m) returns particle m's
-// unshifted position (scaled and shifted by Lp here).
-template <typename Real, int KIND, typename Wrap>
+// unshifted position (scaled and shifted by Lp here).  W2: two partners of
+// a particle per iteration (for the common walk of one partner inside and
+// the next beyond the cutoff, one iteration per particle), same terms in
+// the same order.
+template <typename Real, int KIND, bool W2 = false, typename Wrap>
 __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, double gmin,
                                            const VolProp &vp, bool on, uint32_t start,
                                            uint32_t size, uint32_t a0, uint32_t gapR, bool last,
@@ -784,6 +787,63 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     double xi = X[run ? a0 : 0u];
     double xis = xi * vp.sc;
     double x1 = 0.0, x1s = 0.0;
+    if (W2) {
+    if (__any(run)) {
+#pragma unroll 1
+    do {
+        // partners k and k + 1 of particle t; the second counts only where
+        // the first continued the walk (then k < kR <= w, so the clamp to w
+        // changes no address that counts and keeps the others in the image)
+        const uint32_t i = start + t;
+        const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+        const bool valid = run && k <= kR;
+        const uint32_t k2 = min(k + 1, c.w);
+        const bool cross = t + k >= size, cross2 = t + k2 >= size;
+        const double xj = X[valid ? a0 + t + k + (cross ? gapR : 0u) : 0u];
+        const double xj2 = X[valid ? a0 + t + k2 + (cross2 ? gapR : 0u) : 0u];
+        double xjs = xj * vp.sc, xjs2 = xj2 * vp.sc;
+        if (valid && cross && last) xjs = wrap(t + k - size) * vp.sc + vp.Lp;
+        if (valid && cross2 && last) xjs2 = wrap(t + k2 - size) * vp.sc + vp.Lp;
+        const double d0 = xj - xi, d1 = xjs - xis;
+        const double e0 = xj2 - xi, e1 = xjs2 - xis;
+        const bool isbad = valid && k == 1 && d1 < gmin;
+        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : d1 > q.rc);
+        const bool add = valid && !isbad && !stop;
+        const bool valid2 = add && k < kR;
+        const bool stop2 = valid2 && (sizeof(Real) == 8 ? e0 > q.rc && e1 > q.rc : e1 > q.rc);
+        const bool add2 = valid2 && !stop2;
+        if (sizeof(Real) == 8) {
+            const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
+            const double tt2 = phi64<KIND>(q, e1) - phi64<KIND>(q, e0);
+            part = add ? part + tt : part;
+            part = add2 ? part + tt2 : part;
+        } else {
+            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
+            const float tt2 = phi32<KIND>(q, (float)e1, inside64<KIND>(q, e1));
+            const float y = tt - cc;
+            const float z = sf + y;
+            cc = add ? (z - sf) - y : cc;
+            sf = add ? z : sf;
+            const float y2 = tt2 - cc;
+            const float z2 = sf + y2;
+            cc = add2 ? (z2 - sf) - y2 : cc;
+            sf = add2 ? z2 : sf;
+        }
+        bad |= isbad ? 1u : 0u;
+        x1 = k == 1 ? xj : x1;
+        x1s = k == 1 ? xjs : x1s;
+        // the same particle goes on at k + 2 only if partner k + 1 was
+        // inside and more partners remain
+        const bool nextp = !(add2 && k + 1 < kR);
+        const uint32_t tn = nextp ? t + 1 : t;
+        run = run && !isbad && tn < size;
+        xi = nextp ? x1 : xi;
+        xis = nextp ? x1s : xis;
+        k = nextp ? 1u : k + 2;
+        t = tn;
+    } while (__any(run));
+    }
+    } else {
     if (__any(run)) {
 #pragma unroll 1
     do {
@@ -821,6 +881,7 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         k = nextp ? 1u : k + 1;
         t = tn;
     } while (__any(run));
+    }
     }
     return sizeof(Real) == 8 ? part : (double)sf - (double)cc;
 }
@@ -943,9 +1004,9 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         double vpart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
-            vpart = vol_lane<Real, KIND>(c, q, gmin, vp, vp.valid && act, start, size, a0, gapR,
-                                         c.per && r + 1 == c.B,
-                                         [&](uint32_t m) { return c.R[c.wg + m]; }, bad);
+            vpart = vol_lane<Real, KIND, sizeof(Real) == 4>(
+                c, q, gmin, vp, vp.valid && act, start, size, a0, gapR, c.per && r + 1 == c.B,
+                [&](uint32_t m) { return c.R[c.wg + m]; }, bad);
         }
         {
             double v[2] = {dUs, vpart};
@@ -1359,7 +1420,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         double vpart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
-            vpart = vol_lane<Real, KIND>(
+            vpart = vol_lane<Real, KIND, sizeof(Real) == 4>(
                 c, q, gmin, vp, vp.valid && act && own, start, size, a0, gapR,
                 per && r + 1 == Bs && s + 1 == S,
                 [&](uint32_t m) { return seg_load(&g.pub_head[(uint64_t)sg.first * W + m]); }, bad);
