@@ -1,4 +1,5 @@
-"""Exact NPT averages of 1D nearest-neighbour systems (Takahashi 1942).
+"""Exact NPT averages of 1D nearest-neighbour systems (Takahashi 1942), and
+exact fixed-length averages of two and three particles between walls.
 
 With nearest-neighbour interactions only, the NPT ensemble factorises into
 independent gaps with density w(g) ~ exp(-beta (phi(g) + P g)).
@@ -54,6 +55,51 @@ def walls_npt(N, mom, beta, P):
 def periodic_npt(N, mom):
     m, v = mom["m"], mom["v"]
     return N * m + v / m, mom["ephig"] / m + (N - 1) * mom["ephi"]
+
+
+def walls_nvt_small(N, L, beta, phi, gmin, nbn=-1, breaks=()):
+    """<U> of N = 2 or 3 particles between walls at +-L/2 at fixed L: the
+    configurational integral over the interior gaps, the two wall gaps
+    integrated out as the factor (L - sum of interior gaps).  With N = 3
+    the outer pair (separation g1 + g2) interacts unless nbn is 1."""
+    pts = sorted(b for b in (gmin, *breaks) if gmin <= b <= L)
+    if N == 2:
+        def f(d, k):
+            e = math.exp(-beta * phi(d))
+            return (L - d) * e * (phi(d) if k else 1.0)
+        Z = sum(quad(lambda d: f(d, 0), a, b, limit=400, epsrel=1e-12)[0]
+                for a, b in zip(pts, pts[1:] + [L]) if b > a)
+        EU = sum(quad(lambda d: f(d, 1), a, b, limit=400, epsrel=1e-12)[0]
+                 for a, b in zip(pts, pts[1:] + [L]) if b > a)
+        return EU / Z
+    assert N == 3
+    outer = nbn < 0 or nbn >= 2
+
+    def u3(g1, g2):
+        return phi(g1) + phi(g2) + (phi(g1 + g2) if outer else 0.0)
+
+    def inner(g1, k):
+        hi = L - g1
+        if hi <= gmin:
+            return 0.0
+        cuts = sorted({gmin, hi, *(b for b in breaks if gmin < b < hi),
+                       *(b - g1 for b in breaks if gmin < b - g1 < hi)})
+        tot = 0.0
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            tot += quad(lambda g2: (L - g1 - g2) * math.exp(-beta * u3(g1, g2))
+                        * (u3(g1, g2) if k else 1.0), a, b, limit=200, epsrel=1e-11)[0]
+        return tot
+
+    def outer_int(k):
+        cuts = sorted({gmin, L - gmin, *(b for b in breaks if gmin < b < L - gmin)})
+        return sum(quad(lambda g1: inner(g1, k), a, b, limit=200, epsrel=1e-10)[0]
+                   for a, b in zip(cuts[:-1], cuts[1:]))
+
+    return outer_int(1) / outer_int(0)
+
+
+def yukawa_phi(A=1.0, kappa=0.2, rc=20.0):
+    return lambda g: A * math.exp(-kappa * g) / g if g <= rc else 0.0
 
 
 def hrsw_phi(rod=1.0, eps=1.0, edge=1.5):

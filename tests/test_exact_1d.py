@@ -8,8 +8,8 @@ import numpy as np
 import pytest
 
 from exact1d import (block_mean_sem, gap_moments, harmonic_phi, hrsw_phi, lj_phi, periodic_npt,
-                     walls_npt)
-from helpers import hrsw, lj, oracle_spec
+                     walls_npt, walls_nvt_small, yukawa_phi)
+from helpers import hrsw, lj, oracle_spec, yukawa
 
 from jmmonedmc_amd import _native as nat
 from jmmonedmc_amd.engine import Chain
@@ -107,3 +107,90 @@ def test_engine_matches_exact_npt(gpu, name, form):
     for vals, ex, what in ((Lc, L_ex, "L"), (Uc, U_ex, "U")):
         m, se = vals.mean(), vals.std(ddof=1) / math.sqrt(len(vals))
         assert abs(m - ex) <= 3 * se, (what, m, ex, se)
+
+
+# Two and three particles between walls at fixed length: every pair
+# interacts, so with three particles the outer pair (two places apart) is
+# in the window walk; exact averages by direct integration.
+SHIFT3 = 4 * (3.0 ** -12 - 3.0 ** -6)
+SMALL = {
+    "lj2_walls": lambda seed, cid: (
+        Chain(N=2, L0=5.0, T=1.0, boundary=nat.WALLS, seed=seed, chain_id=cid, **lj()),
+        walls_nvt_small(2, 5.0, 1.0, lj_phi(rc=3.0, shift=SHIFT3), 0.5, breaks=(1.0, 1.5, 3.0))),
+    "lj3_walls": lambda seed, cid: (
+        Chain(N=3, L0=4.0, T=1.0, boundary=nat.WALLS, max_step=0.3, seed=seed, chain_id=cid, **lj()),
+        walls_nvt_small(3, 4.0, 1.0, lj_phi(rc=3.0, shift=SHIFT3), 0.5, breaks=(0.9, 1.12, 1.5, 3.0))),
+    "yukawa3_walls": lambda seed, cid: (
+        Chain(N=3, L0=8.0, T=1.5, boundary=nat.WALLS, seed=seed, chain_id=cid, **yukawa()),
+        walls_nvt_small(3, 8.0, 1 / 1.5, yukawa_phi(), 0.5, breaks=(1.0, 2.0))),
+    "yukawa3_nbn1_walls": lambda seed, cid: (
+        Chain(N=3, L0=8.0, T=1.5, boundary=nat.WALLS, nbn=1, seed=seed, chain_id=cid, **yukawa()),
+        walls_nvt_small(3, 8.0, 1 / 1.5, yukawa_phi(), 0.5, nbn=1, breaks=(1.0, 2.0))),
+}
+
+
+def test_small_walls_integrals_against_closed_form_and_grid():
+    # two hard rods (length 1) with a well of depth eps out to 1.5 in L = 4:
+    # the gap d has weight (L - d) e^{beta eps} on [1, 1.5), (L - d) beyond
+    L, beta, eps = 4.0, 1.0 / 0.7, 1.0
+    w_in = math.exp(beta * eps) * (L * 0.5 - (1.5 ** 2 - 1.0) / 2)
+    w_out = (L - 1.5) ** 2 / 2
+    exact = -eps * w_in / (w_in + w_out)
+    got = walls_nvt_small(2, L, beta, hrsw_phi(1.0, eps, 1.5), 1.0, breaks=(1.5,))
+    assert abs(got - exact) <= 1e-9 * abs(exact)
+    # three Yukawa particles: midpoint sum over the gap triangle
+    L, beta = 8.0, 1 / 1.5
+    phi = yukawa_phi()
+    n = 2400
+    h = (L - 1.0) / n
+    g = 0.5 + h * (np.arange(n) + 0.5)
+    g1, g2 = np.meshgrid(g, g, indexing="ij")
+    ok = g1 + g2 <= L
+    f = lambda d: np.exp(-0.2 * d) / d
+    u = f(g1) + f(g2) + f(g1 + g2)
+    wgt = np.where(ok, (L - g1 - g2) * np.exp(-beta * u), 0.0)
+    grid = float((wgt * u).sum() / wgt.sum())
+    got = walls_nvt_small(3, L, beta, phi, 0.5, breaks=(1.0, 2.0))
+    assert abs(got - grid) <= 2e-4 * abs(grid)
+    # the outer pair matters: dropping it (NBN 1) changes <U> by far more
+    # than the tests' standard errors
+    assert got - walls_nvt_small(3, L, beta, phi, 0.5, nbn=1, breaks=(1.0, 2.0)) > 0.05
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_oracle_matches_exact_small_walls(oracle, name):
+    """12 independent chains of 40000 sweeps; the spread of the per-chain
+    means gives the standard error."""
+    import pyoracle
+    Us, U_ex = [], None
+    for k in range(12):
+        ch, U_ex = SMALL[name](800 + k, k)
+        c = oracle.Chain(oracle_spec(pyoracle, ch))
+        c.sweeps(2000)
+        Us.append(c.sweeps(40000)["U"].mean())
+    vals = np.array(Us)
+    m, se = vals.mean(), vals.std(ddof=1) / math.sqrt(len(vals))
+    assert abs(m - U_ex) <= 3 * se, (m, U_ex, se)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["batched_fp64", "batched_fp32", "plain_fp64"])
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_engine_matches_exact_small_walls(gpu, name, form):
+    """256 independent chains in one engine; the spread of the per-chain
+    means gives the standard error."""
+    from jmmonedmc_amd.engine import Engine
+    chains, U_ex = [], None
+    for k in range(256):
+        ch, U_ex = SMALL[name](2000 + k, k)
+        chains.append(ch)
+    print(f"[gpu] exact small {name} {form}", flush=True)
+    eng = Engine(chains, form=form, sample_capacity=8000, block_sweeps=1000)
+    eng.run(2000)
+    eng.read_samples(8000)
+    eng.run(8000)
+    smp = eng.read_samples(8000)
+    eng.close()
+    Uc = smp["U"].mean(axis=1)
+    m, se = Uc.mean(), Uc.std(ddof=1) / math.sqrt(len(Uc))
+    assert abs(m - U_ex) <= 3 * se, (m, U_ex, se)
