@@ -35,6 +35,26 @@ __device__ __forceinline__ bool nbpos(const double *X, int64_t j, int64_t N, boo
     return true;
 }
 
+// A partner's contribution to the pair-term counter: 1 for each of the old
+// and new separations inside the cutoff (the counter is n_old + n_new).
+template <int KIND>
+__device__ __forceinline__ uint32_t n_inside(const Pot &q, double o, double n)
+{
+    return (inside64<KIND>(q, o) ? 1u : 0u) + (inside64<KIND>(q, n) ? 1u : 0u);
+}
+
+// The same from the flags a walk step has: `in` (the partner is walked),
+// io and inn (inside at the old / new position).  For LJ and Yukawa a
+// walked partner is inside at one position at least (the walk's test is
+// io || inn), so two selects suffice; with a strict cutoff a partner at
+// exactly rc is walked but inside at neither.
+template <int KIND>
+__device__ __forceinline__ uint32_t n_walked(bool in, bool io, bool inn)
+{
+    if (KIND == POT_LJ || KIND == POT_YUKAWA) return in ? ((io && inn) ? 2u : 1u) : 0u;
+    return in ? (io ? 1u : 0u) + (inn ? 1u : 0u) : 0u;
+}
+
 // Energy change for moving the particle at xi (index i) to xn.  Partners:
 // left side nearest first and right side nearest first, each stopping at the
 // first partner beyond the cutoff from both positions or at index distance
@@ -53,7 +73,7 @@ __device__ __forceinline__ double delta_e64(const Pot &q, const double *X, int64
         const double dold = xi - xj, dnew = xn - xj;
         if (dold > q.rc && dnew > q.rc) break;
         dEL += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
-        terms += 2;
+        terms += n_inside<KIND>(q, dold, dnew);
     }
     for (uint32_t k = 1; k <= w; ++k) {
         double xj;
@@ -61,7 +81,7 @@ __device__ __forceinline__ double delta_e64(const Pot &q, const double *X, int64
         const double dold = xj - xi, dnew = xj - xn;
         if (dold > q.rc && dnew > q.rc) break;
         dER += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
-        terms += 2;
+        terms += n_inside<KIND>(q, dold, dnew);
     }
     return dEL + dER;
 }
@@ -79,13 +99,13 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
         if (!nbpos(X, i - (int64_t)k, N, per, L, xj)) break;
         const double dold = xi - xj, dnew = xn - xj;
         if (dold > q.rc && dnew > q.rc) break;
-        const float t = phi32<KIND>(q, (float)dnew, inside64<KIND>(q, dnew)) -
-                        phi32<KIND>(q, (float)dold, inside64<KIND>(q, dold));
+        const bool io = inside64<KIND>(q, dold), in = inside64<KIND>(q, dnew);
+        const float t = phi32<KIND>(q, (float)dnew, in) - phi32<KIND>(q, (float)dold, io);
         const float y = t - c;
         const float tt = s + y;
         c = (tt - s) - y;
         s = tt;
-        terms += 2;
+        terms += (io ? 1u : 0u) + (in ? 1u : 0u);
     }
     const double left = (double)s - (double)c;
     s = 0.0f;
@@ -95,13 +115,13 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
         if (!nbpos(X, i + (int64_t)k, N, per, L, xj)) break;
         const double dold = xj - xi, dnew = xj - xn;
         if (dold > q.rc && dnew > q.rc) break;
-        const float t = phi32<KIND>(q, (float)dnew, inside64<KIND>(q, dnew)) -
-                        phi32<KIND>(q, (float)dold, inside64<KIND>(q, dold));
+        const bool io = inside64<KIND>(q, dold), in = inside64<KIND>(q, dnew);
+        const float t = phi32<KIND>(q, (float)dnew, in) - phi32<KIND>(q, (float)dold, io);
         const float y = t - c;
         const float tt = s + y;
         c = (tt - s) - y;
         s = tt;
-        terms += 2;
+        terms += (io ? 1u : 0u) + (in ? 1u : 0u);
     }
     return left + ((double)s - (double)c);
 }
@@ -595,7 +615,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
                 const bool in = live && !(o > q.rc && nn > q.rc);
                 const double tt = phi64<KIND>(q, nn) - phi64<KIND>(q, o);
                 dS += in ? tt : 0.0;
-                terms += in ? 2u : 0u;
+                terms += in ? n_inside<KIND>(q, o, nn) : 0u;
                 live = in && k < kS;
                 ++k;
             } while (__any(live));
@@ -619,7 +639,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             const float uu = sS + y;
             cS = in ? (uu - sS) - y : cS;
             sS = in ? uu : sS;
-            terms += in ? 2u : 0u;
+            terms += n_walked<KIND>(in, io, inn);
             live = in && k < kS;
             ++k;
         } while (__any(live));
@@ -646,7 +666,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             const double tR = phi64<KIND>(q, nR) - phi64<KIND>(q, oR);
             dEL += inL ? tL : 0.0;
             dER += inR ? tR : 0.0;
-            terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
+            terms += n_walked<KIND>(inL, inside64<KIND>(q, oL), inside64<KIND>(q, nL)) +
+                     n_walked<KIND>(inR, inside64<KIND>(q, oR), inside64<KIND>(q, nR));
             liveL = inL && k < kL;
             liveR = inR && k < kR;
             ++k;
@@ -707,7 +728,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             cR = inR2 ? (uR - sR) - yR : cR;
             sR = inR2 ? uR : sR;
         }
-        terms += (inL ? 2u : 0u) + (inR ? 2u : 0u) + (inL2 ? 2u : 0u) + (inR2 ? 2u : 0u);
+        terms += n_walked<KIND>(inL, ioL, inL64) + n_walked<KIND>(inR, ioR, inR64) +
+                 n_walked<KIND>(inL2, ioL2, inL642) + n_walked<KIND>(inR2, ioR2, inR642);
         liveL = inL2 && k + 1 < kL;
         liveR = inR2 && k + 1 < kR;
         k += 2;
@@ -740,7 +762,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const float uR = sR + yR;
         cR = inR ? (uR - sR) - yR : cR;
         sR = inR ? uR : sR;
-        terms += (inL ? 2u : 0u) + (inR ? 2u : 0u);
+        terms += n_walked<KIND>(inL, ioL, inL64) + n_walked<KIND>(inR, ioR, inR64);
         liveL = inL && k < kL;
         liveR = inR && k < kR;
         ++k;
@@ -1629,7 +1651,7 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
                                 const double dnew = side ? xj - xn : xn - xj;
                                 if (!(dold > q.rc && dnew > q.rc)) {
                                     term += phi64<KIND>(q, dnew) - phi64<KIND>(q, dold);
-                                    tm += 2;
+                                    tm += n_inside<KIND>(q, dold, dnew);
                                 }
                             }
                         }

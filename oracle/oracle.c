@@ -248,6 +248,19 @@ static inline double phi_e(const orc_spec *s, double d, double shift)
     return 0.0;
 }
 
+/* The cutoff test of phi_e: LJ and Yukawa d <= rc, harmonic d < rc, the
+ * square well d < its edge.  The pair-term counter adds one for each of a
+ * partner's old and new separations inside it (n_old + n_new). */
+static inline uint64_t n_inside(const orc_spec *s, double dold, double dnew)
+{
+    double edge = s->potential == ORC_POT_HRSW ? s->pot_p[2] : s->cutoff;
+    int strict = s->potential == ORC_POT_HRSW || s->potential == ORC_POT_HARMONIC;
+    uint64_t n = 0;
+    n += strict ? (dold < edge) : (dold <= edge);
+    n += strict ? (dnew < edge) : (dnew <= edge);
+    return n;
+}
+
 /* ------------------------------------------------------------- schedule */
 
 static double gmin_of(const orc_spec *s)
@@ -432,7 +445,7 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
         double dold = xi - xj, dnew = xn - xj;
         if (dold > rc && dnew > rc) break;
         dEL += phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift);
-        terms += 2;
+        terms += n_inside(s, dold, dnew);
     }
     for (uint64_t k = 1; k <= c->w; ++k) {
         double xj;
@@ -440,7 +453,7 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
         double dold = xj - xi, dnew = xj - xn;
         if (dold > rc && dnew > rc) break;
         dER += phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift);
-        terms += 2;
+        terms += n_inside(s, dold, dnew);
     }
     double dE = dEL + dER;
     c->cnt[C_TERMS] += terms;

@@ -67,6 +67,8 @@ def test_fp32_decisions_and_energy(gpu, oracle, case):
         assert mism.size == 0, f"chain {c}: FP32 decision flips at {mism[:5]}"
         assert np.array_equal(x, ox)
         assert rel(U, oU) <= 1e-5
+        # cutoff decisions are FP64 in both forms: the same pair terms
+        assert cnt["pair_terms"] == orc.counter("pair_terms")
     eng.close()
 
 

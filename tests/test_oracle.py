@@ -293,6 +293,53 @@ def test_trial_energy_change_equals_brute_force(oracle, name):
             assert np.array_equal(x0, x1)
 
 
+def _inside(ch, d):
+    """The cutoff test as the potentials define it: inclusive for LJ and
+    Yukawa, strict for the harmonic spring and the square well's edge."""
+    if ch.potential == nat.POT_HRSW:
+        return d < ch.pot_p[2]
+    if ch.potential == nat.POT_HARMONIC:
+        return d < ch.cutoff
+    return d <= ch.cutoff
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_pair_term_counter_is_n_old_plus_n_new(oracle, name):
+    """The pair-term counter of a trial that reaches the energy sum grows by
+    the partners within the cutoff of the old position plus those within
+    the cutoff of the new one, counted here over every partner (and every
+    periodic image on either side) within the NBN limit."""
+    import pyoracle
+    ch = CASES[name]()[0]
+    c = oracle.Chain(oracle_spec(pyoracle, ch))
+    rng = np.random.default_rng(5)
+    N = ch.N
+    nb = N - 1 if ch.nbn < 0 else min(ch.nbn, N - 1)
+    checked = 0
+    for _ in range(200):
+        i = int(rng.integers(N))
+        md = float(rng.uniform(-ch.max_step, ch.max_step))
+        x, L, _, _ = c.state()
+        before = [c.counter(k) for k in ("pair_terms", "wall_rej", "hc_rej")]
+        c.trial_with(i, md, 0.5)
+        after = [c.counter(k) for k in ("pair_terms", "wall_rej", "hc_rej")]
+        if after[1] != before[1] or after[2] != before[2]:
+            assert after[0] == before[0]
+            continue
+        xi, xn = x[i], x[i] + md
+        want = 0
+        for k in range(1, nb + 1):
+            for j, sign in ((i - k, -1), (i + k, 1)):
+                if ch.boundary == nat.WALLS and not 0 <= j < N:
+                    continue
+                xj = x[j % N] + (L if j >= N else -L if j < 0 else 0.0)
+                o, n = sign * (xj - xi), sign * (xj - xn)
+                want += int(_inside(ch, o)) + int(_inside(ch, n))
+        assert after[0] - before[0] == want, (i, md)
+        checked += 1
+    assert checked > 50
+
+
 def test_accept_rule_hand_cases(oracle):
     L = oracle.lib()
     beta = 1.0 / 0.7
