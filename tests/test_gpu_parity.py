@@ -87,3 +87,32 @@ def test_full_energy_matches_brute_force(gpu, oracle, form):
         assert rel(Uf[c], ub) <= 1e-12
         assert rel(U, ub) <= tol_running
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["c1_periodic_nvt", "c1_walls_truncated"])
+def test_c1_full_length_replays_oracle(gpu, oracle, case):
+    """C1 at its full 1e5 sweeps (6.4e6 trials), batched FP64: every
+    sweep's accepted-move count, the final positions bit for bit, the
+    running energy and every counter equal to the oracle's."""
+    sweeps, chunk = 100_000, 10_000
+    ch = CASES[case]()[0]
+    print(f"[gpu] C1 full length {case}", flush=True)
+    eng = Engine([ch], form="batched_fp64", sample_capacity=chunk, block_sweeps=1000)
+    acc = []
+    for _ in range(sweeps // chunk):
+        eng.run(chunk)
+        acc.append(eng.read_samples(chunk)[0]["acc_d"].copy())
+    x, L, U, sw = eng.state(0)
+    cnt = eng.counters(0)
+    eng.close()
+    orc = oracle.Chain(oracle_spec(oracle, ch))
+    osmp = orc.sweeps(sweeps)
+    ox, oL, oU, osw = orc.state()
+    assert sw == osw == sweeps
+    np.testing.assert_array_equal(np.concatenate(acc), osmp["acc_d"])
+    assert np.array_equal(x, ox) and L == oL
+    assert rel(U, oU) <= 1e-10
+    for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("wall_rejects", "wall_rej"),
+                  ("core_rejects", "hc_rej"), ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
+        assert cnt[k] == orc.counter(ok), k
