@@ -116,3 +116,35 @@ def test_c1_full_length_replays_oracle(gpu, oracle, case):
     for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("wall_rejects", "wall_rej"),
                   ("core_rejects", "hc_rej"), ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
         assert cnt[k] == orc.counter(ok), k
+
+
+@pytest.mark.gpu
+def test_c2_chain_full_length_replays_oracle(gpu, oracle):
+    """One C2 chain (4096 particles, NPT on a ring, ln L moves) at the
+    configuration's full 1e4 sweeps, batched FP64: every sweep's accepts and
+    box length, the final positions bit for bit, every counter."""
+    from jmmonedmc_amd import configs
+    sweeps, chunk = 10_000, 2_500
+    ch = configs.c2(n_chains=1)[0]
+    print("[gpu] C2 chain full length", flush=True)
+    eng = Engine([ch], form="batched_fp64", sample_capacity=chunk, block_sweeps=500)
+    parts = []
+    for _ in range(sweeps // chunk):
+        eng.run(chunk)
+        parts.append(eng.read_samples(chunk)[0].copy())
+    smp = np.concatenate(parts)
+    x, L, U, sw = eng.state(0)
+    cnt = eng.counters(0)
+    eng.close()
+    orc = oracle.Chain(oracle_spec(oracle, ch))
+    osmp = orc.sweeps(sweeps)
+    ox, oL, oU, osw = orc.state()
+    np.testing.assert_array_equal(smp["acc_d"], osmp["acc_d"])
+    np.testing.assert_array_equal(smp["acc_v"], osmp["acc_v"])
+    np.testing.assert_array_equal(smp["L"], osmp["L"])
+    assert osmp["acc_v"].sum() > 0
+    assert np.array_equal(x, ox) and L == oL
+    assert rel(U, oU) <= 1e-10
+    for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"), ("core_rejects", "hc_rej"),
+                  ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
+        assert cnt[k] == orc.counter(ok), k
