@@ -148,3 +148,49 @@ def test_c2_chain_full_length_replays_oracle(gpu, oracle):
     for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"), ("core_rejects", "hc_rej"),
                   ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
         assert cnt[k] == orc.counter(ok), k
+
+
+@pytest.mark.gpu
+def test_c3_sample_full_length_replays_oracle(gpu, oracle):
+    """Six chains of the C3 workload (the smallest, median and largest of
+    each potential, N = 32 to 8176) in one engine at the workload's full 1e4
+    sweeps, batched FP64: every sweep's accepts and box length, the final
+    positions bit for bit, every counter."""
+    import concurrent.futures as cf
+    from jmmonedmc_amd import _native as nat
+    from jmmonedmc_amd import configs
+    sweeps, chunk = 10_000, 2_500
+    c3 = configs.c3()
+    pick = []
+    for p in (nat.POT_LJ, nat.POT_HRSW):
+        idx = sorted((i for i, c in enumerate(c3) if c.potential == p), key=lambda i: c3[i].N)
+        pick += [idx[0], idx[len(idx) // 2], idx[-1]]
+    chains = [c3[i] for i in pick]
+    print(f"[gpu] C3 sample full length N={[c.N for c in chains]}", flush=True)
+    eng = Engine(chains, form="batched_fp64", sample_capacity=chunk, block_sweeps=500)
+    parts = []
+    for _ in range(sweeps // chunk):
+        eng.run(chunk)
+        parts.append(eng.read_samples(chunk).copy())
+    smp = np.concatenate(parts, axis=1)
+    states = [eng.state(c) for c in range(len(chains))]
+    cnts = [eng.counters(c) for c in range(len(chains))]
+    eng.close()
+
+    def replay(ch):
+        orc = oracle.Chain(oracle_spec(oracle, ch))
+        return orc, orc.sweeps(sweeps)
+
+    with cf.ThreadPoolExecutor(len(chains)) as ex:
+        ref = list(ex.map(replay, chains))
+    for c, (orc, osmp) in enumerate(ref):
+        x, L, U, sw = states[c]
+        ox, oL, oU, osw = orc.state()
+        np.testing.assert_array_equal(smp[c]["acc_d"], osmp["acc_d"])
+        np.testing.assert_array_equal(smp[c]["acc_v"], osmp["acc_v"])
+        np.testing.assert_array_equal(smp[c]["L"], osmp["L"])
+        assert np.array_equal(x, ox) and L == oL, c
+        assert rel(U, oU) <= 1e-10
+        for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"),
+                      ("core_rejects", "hc_rej"), ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
+            assert cnts[c][k] == orc.counter(ok), (c, k)
