@@ -175,6 +175,33 @@ def test_c5_chain_replays_oracle(gpu, oracle):
     assert np.array_equal(smp["acc_v"], osmp["acc_v"])
 
 
+def test_c5_chain_long_replays_oracle(gpu, oracle):
+    """The same split C5 chain for 200 sweeps in two 100-sweep launches
+    (about 600 neighbour syncs per segment, box moves accepted), FP64 form:
+    every sweep's accepts and box length, the final positions bit for bit,
+    every counter equal to the oracle's."""
+    from jmmonedmc_amd.configs import c5
+    ch = c5(1)[0]
+    print("[gpu] split oracle c5 chain N=65536, 200 sweeps", flush=True)
+    with Engine([ch], form="batched_fp64", sample_capacity=200, block_sweeps=100) as e:
+        assert e.layout(0)["mode"] == "split"
+        e.run(200)
+        smp = e.read_samples(200)[0]
+        x, L, U, sw = e.state(0)
+        cnt = e.counters(0)
+    orc = oracle.Chain(oracle_spec(oracle, ch))
+    osmp = orc.sweeps(200)
+    ox, oL, oU, osw = orc.state()
+    assert np.array_equal(smp["acc_d"], osmp["acc_d"])
+    assert np.array_equal(smp["acc_v"], osmp["acc_v"]) and osmp["acc_v"].sum() > 0
+    assert np.array_equal(smp["L"], osmp["L"])
+    assert np.array_equal(x, ox) and L == oL and sw == osw == 200
+    assert rel(U, oU) <= 1e-10
+    for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"), ("core_rejects", "hc_rej"),
+                  ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
+        assert cnt[k] == orc.counter(ok), k
+
+
 def test_every_placement_in_one_engine_replays_oracle(gpu, oracle):
     """One engine holding a chain that runs from HBM (24000 particles with
     one block: too large for a workgroup and nothing to split), a chain in
