@@ -1159,11 +1159,12 @@ k_sweep_batched(SweepArgs a)
 
 // ---------------------------------------------------------- segmented chains
 
-// Synchronisation between the workgroups (segments) of one chain.  Every
-// wave of a segment fences its boundary stores at agent scope before the
-// workgroup barrier; thread 0 then advances the segment's counter.  A waiter
-// polls its partners' counters with agent-scope loads and fences before the
-// workgroup reads what they published.  Every wait is bounded: after 2^22
+// Synchronisation between the workgroups (segments) of one chain.  A
+// workgroup barrier brings every wave's boundary stores to the CU's L2;
+// thread 0 then fences at agent scope (one write-back of that L2 per
+// workgroup, where a fence in every wave cost one per wave) and advances the
+// segment's counter.  A waiter polls its partners' counters with agent-scope
+// loads and fences before the workgroup reads what they published.  Every wait is bounded: after 2^22
 // polls the error flag is set and all later waits return at once, so the
 // grid drains in bounded time and the host reports the error.
 __device__ __forceinline__ void seg_signal(unsigned long long *ctr, unsigned long long v)
@@ -1308,7 +1309,6 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
     };
     auto sync_nb = [&]() {
         ++sp;
-        __threadfence();
         __syncthreads();
         if (tid == 0) {
             seg_signal(&g.ctr[me], sp);
@@ -1319,7 +1319,6 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
     };
     auto sync_all = [&]() {
         ++sp;
-        __threadfence();
         __syncthreads();
         if (tid == 0) {
             seg_signal(&g.ctr[me], sp);
