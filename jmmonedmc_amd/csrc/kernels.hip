@@ -1164,7 +1164,8 @@ k_sweep_batched(SweepArgs a)
 // thread 0 then fences at agent scope (one write-back of that L2 per
 // workgroup, where a fence in every wave cost one per wave) and advances the
 // segment's counter.  A waiter polls its partners' counters with agent-scope
-// loads and fences before the workgroup reads what they published.  Every wait is bounded: after 2^22
+// loads and, once all have arrived, fences once before the workgroup reads
+// what they published.  Every wait is bounded: after 2^22
 // polls the error flag is set and all later waits return at once, so the
 // grid drains in bounded time and the host reports the error.
 __device__ __forceinline__ void seg_signal(unsigned long long *ctr, unsigned long long v)
@@ -1178,10 +1179,7 @@ __device__ __noinline__ void seg_wait(const unsigned long long *ctr, unsigned lo
 {
     if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     for (uint32_t it = 0; it < (1u << 22); ++it) {
-        if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
-            __threadfence();
-            return;
-        }
+        if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return;
         __builtin_amdgcn_s_sleep(2);
     }
     __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1314,6 +1312,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
             seg_signal(&g.ctr[me], sp);
             if (hasLn) seg_wait(&g.ctr[ln], sp, g.err);
             if (hasRn) seg_wait(&g.ctr[rn], sp, g.err);
+            __threadfence();
         }
         __syncthreads();
     };
@@ -1324,6 +1323,7 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
             seg_signal(&g.ctr[me], sp);
             for (uint32_t j = 0; j < S; ++j)
                 if (j != s) seg_wait(&g.ctr[sg.first + j], sp, g.err);
+            __threadfence();
         }
         __syncthreads();
     };
