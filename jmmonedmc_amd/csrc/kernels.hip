@@ -625,6 +625,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             return go && (dE <= 0.0 || bf > u);
         }
         float sS = 0.0f, cS = 0.0f;
+        int wS = 0;  // hard rod + square well: the well count, as the one-lane walk
         uint32_t k = 1;
         if (__any(live)) {
 #pragma unroll 1
@@ -635,16 +636,20 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             const bool io = inside64<KIND>(q, o), inn = inside64<KIND>(q, nn);
             const bool in = live && walk_in<KIND>(q, o, nn, io, inn);
             const float tt = phi32<KIND>(q, (float)nn, inn) - phi32<KIND>(q, (float)o, io);
-            const float y = tt - cS;
-            const float uu = sS + y;
-            cS = in ? (uu - sS) - y : cS;
-            sS = in ? uu : sS;
+            if (KIND == POT_HRSW) {
+                wS += in ? (int)inn - (int)io : 0;
+            } else {
+                const float y = tt - cS;
+                const float uu = sS + y;
+                cS = in ? (uu - sS) - y : cS;
+                sS = in ? uu : sS;
+            }
             terms += n_walked<KIND>(in, io, inn);
             live = in && k < kS;
             ++k;
         } while (__any(live));
         }
-        const double mine = (double)sS - (double)cS;
+        const double mine = KIND == POT_HRSW ? -q.p1 * (double)wS : (double)sS - (double)cS;
         dE = mine + __shfl_xor(mine, 1);
         const float bf = fexp(-betaf * (float)dE);
         return go && (dE <= 0.0 || (double)bf > u);
@@ -678,6 +683,11 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         return go && (dE <= 0.0 || bf > u);
     }
     float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
+    // hard rod + square well: every pair term is -eps or 0, so each side's
+    // change is -eps times the change in the number of partners inside the
+    // well, counted exactly (for eps = 1 the compensated FP32 sums give the
+    // same value)
+    int wL = 0, wR = 0;
     uint32_t k = 1;
     if (W2) {
     if (__any(liveL || liveR)) {
@@ -708,6 +718,10 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const bool inR2 = liveR2 && walk_in<KIND>(q, oR2, nR2, ioR2, inR642);
         const float tL2 = phi32<KIND>(q, (float)nL2, inL642) - phi32<KIND>(q, (float)oL2, ioL2);
         const float tR2 = phi32<KIND>(q, (float)nR2, inR642) - phi32<KIND>(q, (float)oR2, ioR2);
+        if (KIND == POT_HRSW) {
+            wL += (inL ? (int)inL64 - (int)ioL : 0) + (inL2 ? (int)inL642 - (int)ioL2 : 0);
+            wR += (inR ? (int)inR64 - (int)ioR : 0) + (inR2 ? (int)inR642 - (int)ioR2 : 0);
+        } else {
         {
             const float yL = tL - cL;
             const float uL = sL + yL;
@@ -727,6 +741,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             const float uR = sR + yR;
             cR = inR2 ? (uR - sR) - yR : cR;
             sR = inR2 ? uR : sR;
+        }
         }
         terms += n_walked<KIND>(inL, ioL, inL64) + n_walked<KIND>(inR, ioR, inR64) +
                  n_walked<KIND>(inL2, ioL2, inL642) + n_walked<KIND>(inR2, ioR2, inR642);
@@ -753,15 +768,20 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const bool inR = liveR && walk_in<KIND>(q, oR, nR, ioR, inR64);
         const float tL = phi32<KIND>(q, (float)nL, inL64) - phi32<KIND>(q, (float)oL, ioL);
         const float tR = phi32<KIND>(q, (float)nR, inR64) - phi32<KIND>(q, (float)oR, ioR);
-        // compensated sums; a side whose walk has ended keeps its sum
-        const float yL = tL - cL;
-        const float uL = sL + yL;
-        cL = inL ? (uL - sL) - yL : cL;
-        sL = inL ? uL : sL;
-        const float yR = tR - cR;
-        const float uR = sR + yR;
-        cR = inR ? (uR - sR) - yR : cR;
-        sR = inR ? uR : sR;
+        if (KIND == POT_HRSW) {
+            wL += inL ? (int)inL64 - (int)ioL : 0;
+            wR += inR ? (int)inR64 - (int)ioR : 0;
+        } else {
+            // compensated sums; a side whose walk has ended keeps its sum
+            const float yL = tL - cL;
+            const float uL = sL + yL;
+            cL = inL ? (uL - sL) - yL : cL;
+            sL = inL ? uL : sL;
+            const float yR = tR - cR;
+            const float uR = sR + yR;
+            cR = inR ? (uR - sR) - yR : cR;
+            sR = inR ? uR : sR;
+        }
         terms += n_walked<KIND>(inL, ioL, inL64) + n_walked<KIND>(inR, ioR, inR64);
         liveL = inL && k < kL;
         liveR = inR && k < kR;
@@ -769,7 +789,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     } while (__any(liveL || liveR));
     }
     }
-    dE = ((double)sL - (double)cL) + ((double)sR - (double)cR);
+    dE = KIND == POT_HRSW ? -q.p1 * (double)wL + -q.p1 * (double)wR
+                          : ((double)sL - (double)cL) + ((double)sR - (double)cR);
     const float bf = fexp(-betaf * (float)dE);
     return go && (dE <= 0.0 || (double)bf > u);
 }
@@ -791,7 +812,9 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 // scaled position are kept rather than read again.  `last`: the lane holds
 // the chain's last block of a periodic chain, whose partners past the end
 // are the first particles shifted by L; wrap(m) returns particle m's
-// unshifted position (scaled and shifted by Lp here).  W2: two partners of
+// unshifted position (scaled and shifted by Lp here).  The hard-rod +
+// square-well FP32 energy is -eps times the number of pairs inside the well,
+// counted exactly.  W2: two partners of
 // a particle per iteration (for the common walk of one partner inside and
 // the next beyond the cutoff, one iteration per particle), same terms in
 // the same order.
@@ -804,6 +827,7 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     const double *X = c.R;
     double part = 0.0;
     float sf = 0.0f, cc = 0.0f;
+    int wc = 0;  // hard rod + square well, FP32: pairs inside the well (energy -eps each)
     bool run = on && size > 0;
     uint32_t t = 0, k = 1;
     double xi = X[run ? a0 : 0u];
@@ -839,6 +863,8 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
             const double tt2 = phi64<KIND>(q, e1) - phi64<KIND>(q, e0);
             part = add ? part + tt : part;
             part = add2 ? part + tt2 : part;
+        } else if (KIND == POT_HRSW) {
+            wc += (add && inside64<KIND>(q, d1) ? 1 : 0) + (add2 && inside64<KIND>(q, e1) ? 1 : 0);
         } else {
             const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
             const float tt2 = phi32<KIND>(q, (float)e1, inside64<KIND>(q, e1));
@@ -883,6 +909,8 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         if (sizeof(Real) == 8) {
             const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             part = add ? part + tt : part;
+        } else if (KIND == POT_HRSW) {
+            wc += add && inside64<KIND>(q, d1) ? 1 : 0;
         } else {
             const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
             const float y = tt - cc;
@@ -905,7 +933,8 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     } while (__any(run));
     }
     }
-    return sizeof(Real) == 8 ? part : (double)sf - (double)cc;
+    if (sizeof(Real) == 8) return part;
+    return KIND == POT_HRSW ? -q.p1 * (double)wc : (double)sf - (double)cc;
 }
 
 template <typename Real, int KIND, int T>
