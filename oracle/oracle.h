@@ -23,7 +23,7 @@
  * taus2 stream); they are restated here so the oracle replays the engine move
  * for move.
  *
- * Parity pinning, per function (tests/test_oracle_*.py):
+ * Parity pinning, per function (tests/test_oracle.py, tests/test_exact_1d.py):
  *   orc_philox4x32_10  Random123 known-answer vectors + rocRAND's
  *                      philox4x32_10 engine on the host
  *   orc_phi            closed forms (LJ zero at sigma, minimum -eps at
@@ -31,13 +31,17 @@
  *                      inclusive/strict cutoffs, harmonic penalty) and
  *                      central finite differences for the virial/gradients
  *   orc_trial          brute-force O(N^2) recompute before/after on tiny N;
- *                      hand-chosen (x, md, u) accept triples
+ *                      hand-chosen (x, md, u) accept triples; its pair-term
+ *                      counter against n_old + n_new counted over every
+ *                      partner and periodic image
  *   orc_full_energy    brute-force all-pairs sum (orc_brute_energy, which has
  *                      no window logic) on random and lattice inputs
  *   orc_volume_trial   hand-chosen (L, dl, dU, u) acceptance cases and the
  *                      LJ scaling identity for an infinite cutoff
  *   orc_sweep          exact 1D NPT averages (Takahashi gas of independent
- *                      gaps, integrated by quadrature) within 3 standard
+ *                      gaps, integrated by quadrature) and the exact
+ *                      averages of two and three particles between walls
+ *                      (configurational integrals) within 3 standard
  *                      errors; incremental energy == full recompute
  *   orc_observables    central finite differences in theta and x
  *   orc_det_exp        libm exp to 2 ulp over the move range
