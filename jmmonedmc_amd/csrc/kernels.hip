@@ -536,14 +536,17 @@ __device__ __forceinline__ void fill_ghosts(const LdsChain &c, uint32_t r, doubl
     }
 }
 
-// The walk's keep-going test for one partner: not beyond the cutoff from
-// both the old and the new position (FP64, the FP64 form's rule).  For the
-// potentials whose inside test is d <= rc it is the inside flags' or.
+// The FP32 walks' keep-going test for one partner: inside the cutoff at the
+// old or the new position (the FP64 flags).  For LJ and Yukawa (inside is
+// d <= rc) this is the FP64 form's rule, not beyond the cutoff from both
+// positions.  For the strict cutoffs of the square well and the spring the
+// two differ only for a partner at exactly rc: it adds nothing to the sums
+// or the pair-term count, and the FP64 walk's next partner lies beyond rc
+// and adds nothing either, so both walks give the same result.
 template <int KIND>
-__device__ __forceinline__ bool walk_in(const Pot &q, double o, double n, bool io, bool in)
+__device__ __forceinline__ bool walk_in(const Pot &, double, double, bool io, bool in)
 {
-    if (KIND == POT_LJ || KIND == POT_YUKAWA) return io || in;
-    return !(o > q.rc && n > q.rc);
+    return io || in;
 }
 
 // One displacement trial of particle i (block position t of a block of
