@@ -49,6 +49,11 @@ def parse():
     p.add_argument("--graph", type=int, default=1)
     p.add_argument("--tune", type=int, default=1,
                    help="choose the batched forms' launch parameters with the autotuner first")
+    p.add_argument("--acc-burnin", type=int, default=-1,
+                   help="FP64 sweeps that equilibrate the common start of the accuracy runs "
+                        "(-1 = the workload's sweep count: 1e4, C5 2e3)")
+    p.add_argument("--acc-sweeps", type=int, default=-1,
+                   help="sweeps of each accuracy run (0 = no accuracy check, -1 = as --acc-burnin)")
     return p.parse_args()
 
 
@@ -81,14 +86,6 @@ def algorithmic_flops(eng, chains, before):
     tp = np.array([trans_per_pair(ch.potential) for ch in chains], dtype=np.float64)
     trans = float((tp * terms + trials).sum())
     return tot, int(terms.sum()), int(trials.sum()), trans
-
-
-def block_sem(x, nblocks=20):
-    n = len(x) // nblocks * nblocks
-    if n < nblocks:
-        return float("nan")
-    m = x[:n].reshape(nblocks, -1).mean(axis=1)
-    return float(m.std() / math.sqrt(nblocks))
 
 
 def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=False):
@@ -136,6 +133,67 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
                            "includes": "kernel and read-back of U, W, dU/dtheta, dW/dtheta and "
                                        "dU/dx for every chain"}
     return out
+
+
+GATE = ("U", "L", "rho")
+
+
+def accuracy_check(chains, opts, burnin, sweeps, bs):
+    """FP32 against FP64 per chain and observable at stated accuracy.
+
+    A burn-in of `burnin` FP64 sweeps from the lattice start gives every
+    chain a common starting configuration; three runs of `sweeps` sweeps
+    start from it with independent seeds: an FP64 control, a second FP64
+    run (the null) and the FP32 form.  Per chain and run: <U>, <L>, <rho>
+    with standard errors sqrt(2 tau_int var / n), the fluctuation
+    properties cP and betaT (block formulas), and tau_int of U.  z-scores of
+    the null and of the FP32 run against the control, over the chains whose
+    estimates rest on at least 30 independent samples (the rest counted as
+    excluded).  The gate is valid when the null has at least 99 % of its
+    chains within 3 standard errors for <U>, <L> and <rho>; it passes when
+    the FP32 run does too.  The FP32 error per observable is stated as its
+    median and RMS relative difference from the control beside the null's
+    (the resolution of the comparison)."""
+    import dataclasses
+    from jmmonedmc_amd import stats
+    from jmmonedmc_amd.engine import Engine
+    t0 = time.perf_counter()
+    print(f"[bench] accuracy: {burnin} FP64 burn-in sweeps", file=sys.stderr, flush=True)
+    with Engine(chains, form="batched_fp64", sample_capacity=bs, block_sweeps=bs, **opts) as e:
+        for _ in range(max(1, burnin // bs)):
+            e.run(bs)
+            e.read_samples(bs)
+        x, L0, _ = e.states()
+    N = np.array([c.N for c in chains])
+    P = np.array([c.P for c in chains])
+    T = np.array([c.T for c in chains])
+    obs = {}
+    for name, form, off in (("fp64_control", "batched_fp64", 1_000_003), ("fp64_null", "batched_fp64", 2_000_003),
+                            ("fp32", "batched_fp32", 3_000_003)):
+        print(f"[bench] accuracy: {name} run of {sweeps} sweeps", file=sys.stderr, flush=True)
+        chs = [dataclasses.replace(c, seed=c.seed + off) for c in chains]
+        with Engine(chs, form=form, sample_capacity=sweeps, block_sweeps=bs, **opts) as e:
+            e.set_states(x, L0)
+            e.run(sweeps)
+            smp = e.read_samples(sweeps)
+        obs[name] = stats.chain_observables_batch(smp["U"], smp["L"], N, P, T)
+        del smp
+    null = stats.compare_runs(obs["fp64_control"], obs["fp64_null"])
+    fp32 = stats.compare_runs(obs["fp64_control"], obs["fp32"])
+    valid = all(null[k]["frac_within_3sigma"] is not None and null[k]["frac_within_3sigma"] >= 0.99
+                for k in GATE)
+    passed = valid and all(fp32[k]["frac_within_3sigma"] is not None and fp32[k]["frac_within_3sigma"] >= 0.99
+                           for k in GATE)
+    per_obs = {}
+    for k in stats.OBSERVABLES:
+        per_obs[k] = {"fp32_vs_fp64": fp32[k], "null_fp64_vs_fp64": null[k]}
+    return {"method": "common start after an FP64 burn-in, three runs with independent seeds (FP64 control, "
+                      "FP64 null, FP32); per chain, mean with SE sqrt(2 tau_int var/n); z against the control "
+                      "over chains with >= 30 independent samples",
+            "burnin_sweeps": burnin, "sweeps_per_run": sweeps, "chains": len(chains),
+            "gate_observables": list(GATE), "gate_valid_null_ge_0.99": bool(valid),
+            "gate_passed": bool(passed), "observables": per_obs,
+            "seconds": time.perf_counter() - t0}
 
 
 def cpu_baseline(chains, sweeps, threads, allpairs=False):
@@ -194,6 +252,8 @@ def main():
     forms = set(a.forms.split(","))
     chains, desc = workload(a.workload)
     n_trials_per_sweep = sum(ch.N for ch in chains)
+    T0 = time.perf_counter()
+    timeline = []
 
     tuned = None
     opts = {}
@@ -203,64 +263,38 @@ def main():
         tuned = tune(chains, form="batched_fp32", block_sweeps=a.block_sweeps)
         opts = dict(tuned.best)
     bwd = a.workload == "c4"
-    fp32 = run_form(chains, "batched_fp32", a.steps, a.warmup, a.block_sweeps, a.graph, "batched_fp32",
-                    opts, backward=bwd)
+
+    def leg(name, fn):
+        t = time.perf_counter() - T0
+        r = fn()
+        timeline.append({"leg": name, "start_s": round(t, 3), "end_s": round(time.perf_counter() - T0, 3)})
+        return r
+
+    if tuned:
+        timeline.append({"leg": "autotune (GPU)", "start_s": 0.0, "end_s": round(time.perf_counter() - T0, 3)})
+    fp32 = leg("batched_fp32 (GPU, the value)",
+               lambda: run_form(chains, "batched_fp32", a.steps, a.warmup, a.block_sweeps, a.graph,
+                                "batched_fp32", opts, backward=bwd))
+    samples_bytes = int(fp32["samples"].size * 24)
+    fp32.pop("samples")
     res = {}
     if "fp64" in forms:
-        res["batched_fp64"] = run_form(chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps,
-                                       a.graph, "batched_fp64", opts, backward=bwd)
+        res["batched_fp64"] = leg("batched_fp64 (GPU)",
+                                  lambda: run_form(chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps,
+                                                   a.graph, "batched_fp64", opts, backward=bwd))
+        res["batched_fp64"].pop("samples")
     if "plain" in forms and a.plain_steps > 0:
-        res["plain_fp64"] = run_form(chains, "plain_fp64", a.plain_steps, 0, a.block_sweeps, 0,
-                                     "plain_fp64")
+        res["plain_fp64"] = leg("plain_fp64 (GPU)",
+                                lambda: run_form(chains, "plain_fp64", a.plain_steps, 0, a.block_sweeps, 0,
+                                                 "plain_fp64"))
+        res["plain_fp64"].pop("samples")
 
-    # <U> agreement.  Chains with the same seed share their random numbers,
-    # so FP32 and FP64 runs of one seed stay correlated; the comparison is
-    # made against an FP64 control run with independent seeds, and the same
-    # statistic between the two FP64 runs gives the null distribution (slowly
-    # equilibrating chains widen both alike).
     accuracy = None
-    if "batched_fp64" in res:
-        import dataclasses
-        ctl_chains = [dataclasses.replace(ch, seed=ch.seed + 1_000_003) for ch in chains]
-        ctl = run_form(ctl_chains, "batched_fp64", a.steps, a.warmup, a.block_sweeps, a.graph,
-                       "batched_fp64 control (independent seeds, untimed for the value)", opts)
-        U_ctl = ctl["samples"]["U"]
-        h = U_ctl.shape[1] // 2
-
-        def zscores(Ua):
-            z = np.full(len(chains), np.nan)
-            for c in range(len(chains)):
-                se = math.hypot(block_sem(Ua[c, h:]), block_sem(U_ctl[c, h:]))
-                if se > 0:
-                    z[c] = (Ua[c, h:].mean() - U_ctl[c, h:].mean()) / se
-            return z
-
-        z32a = zscores(fp32["samples"]["U"])
-        z64a = zscores(res["batched_fp64"]["samples"]["U"])
-        z32, z64 = z32a[np.isfinite(z32a)], z64a[np.isfinite(z64a)]
-        # chains whose control run is stationary over the compared half
-        # (third and fourth quarter means within 3 combined SE): slowly
-        # equilibrating chains drift there and their block SE understates
-        # the error in both comparisons
-        q = U_ctl.shape[1] // 4
-        stat = np.zeros(len(chains), dtype=bool)
-        for c in range(len(chains)):
-            a3, a4 = U_ctl[c, 2 * q:3 * q], U_ctl[c, 3 * q:]
-            se = math.hypot(block_sem(a3, 10), block_sem(a4, 10))
-            stat[c] = se > 0 and abs(a3.mean() - a4.mean()) <= 3 * se
-        z32s, z64s = z32a[stat & np.isfinite(z32a)], z64a[stat & np.isfinite(z64a)]
-        accuracy = {"observable": "<U> per chain over the second half of the run, 20-block SEM, "
-                                  "against an FP64 run with independent seeds",
-                    "chains_compared": int(z32.size),
-                    "frac_within_3sigma": float((np.abs(z32) <= 3).mean()),
-                    "null_frac_within_3sigma_fp64_vs_fp64": float((np.abs(z64) <= 3).mean()),
-                    "mean_z": float(z32.mean()), "null_mean_z": float(z64.mean()),
-                    "std_z": float(z32.std()), "null_std_z": float(z64.std()),
-                    "max_abs_z": float(np.abs(z32).max()),
-                    "null_max_abs_z": float(np.abs(z64).max()),
-                    "stationary_chains": int(z32s.size),
-                    "stationary_frac_within_3sigma": float((np.abs(z32s) <= 3).mean()) if z32s.size else None,
-                    "stationary_null_frac_within_3sigma": float((np.abs(z64s) <= 3).mean()) if z64s.size else None}
+    acc_burn = a.acc_burnin if a.acc_burnin >= 0 else (2000 if a.workload == "c5" else 10000)
+    acc_sw = a.acc_sweeps if a.acc_sweeps >= 0 else acc_burn
+    if acc_sw > 0:
+        accuracy = leg("accuracy runs (GPU, untimed)",
+                       lambda: accuracy_check(chains, opts, acc_burn, acc_sw, a.block_sweeps))
 
     # roofline of the dominant kernel (the batched FP32 sweep kernel)
     achieved = fp32["flops"] / (fp32["kernel_ms"] * 1e-3) / 1e12
@@ -281,13 +315,20 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        sw = a.cpu_sweeps or max(1, int(15.0 * 1.2e7 * threads / n_trials_per_sweep))
-        v, dt, tr = cpu_baseline(chains, sw, threads)
+        # every CPU the process may run on; a host that allots a CPU share per
+        # GPU job states it in OMP_NUM_THREADS, which then caps the threads
+        host = os.cpu_count() or 1
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else host
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = min(avail, omp) if omp > 0 else avail
+        sw = a.cpu_sweeps or max(1, int(12.0 * 1.2e7 * threads / n_trials_per_sweep))
+        v, dt, tr = leg("cpu baseline, all threads (CPU)", lambda: cpu_baseline(chains, sw, threads))
         cpu = {"value": v, "unit": "trial moves/s", "cores": threads, "kind": "port",
+               "host_cpus": host, "cpus_available": avail, "omp_num_threads": omp or None,
                "sample": f"FP64 oracle, all {len(chains)} chains x {sw} sweeps ({tr:.3g} trials, "
-                         f"{dt:.1f} s), one chain per thread"}
-        cpu.update(cpu_extra(chains, n_trials_per_sweep))
+                         f"{dt:.1f} s), one chain per thread on {threads} threads"}
+        cpu.update(leg("cpu baseline, one core and the all-partner algorithm (CPU)",
+                       lambda: cpu_extra(chains, n_trials_per_sweep)))
 
     plain = res.get("plain_fp64")
     out = {
@@ -310,14 +351,21 @@ def main():
         "autotune_ms_per_sweep": tuned and tuned.ms_per_sweep,
         "chain_sweeps_per_s": len(chains) * a.steps * a.block_sweeps / fp32["wall_s"],
         "sample_readback": {"seconds": fp32["readback_s"],
-                            "bytes": int(fp32["samples"].size * 24),
+                            "bytes": samples_bytes,
                             "value_including_readback": fp32["trials"] / (fp32["wall_s"] + fp32["readback_s"])},
         "plain_fp64": plain and {"value": plain["trials_per_s"], "sweeps_timed": plain["sweeps"],
                                  "ms_per_block": plain["ms_per_step"]},
         "speedup_vs_plain_fp64": plain and fp32["trials_per_s"] / plain["trials_per_s"],
+        "speedup_vs_batched_fp64": ("batched_fp64" in res) and
+                                   fp32["trials_per_s"] / res["batched_fp64"]["trials_per_s"] or None,
+        "value_status": (None if accuracy is None else
+                         "accuracy gate passed" if accuracy["gate_passed"] else
+                         "accuracy gate failed" if accuracy["gate_valid_null_ge_0.99"] else
+                         "accuracy gate not valid (the FP64 null is below 99 % within 3 sigma)"),
         "roofline": roof,
         "cpu_baseline": cpu,
         "accuracy_fp32_vs_fp64": accuracy,
+        "timeline": timeline,
     }
     if "backward" in fp32:
         out["backward"] = fp32["backward"]

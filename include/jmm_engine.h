@@ -96,7 +96,10 @@ typedef struct jmm_engine_opts {
     uint32_t seg_lanes;       /* split chains: lanes per block, 1 or 2 (two lanes share each
                                  trial, one per side; 0 = 1); results are the same bits
                                  either way */
-    uint32_t reserved[2];
+    uint32_t sample_virial;   /* 1: after every sweep each chain's virial W = sum -d phi'(d)
+                                 and hypervirial sum -d W'(d) are computed (FP64) and kept
+                                 with the samples (jmm_read_samples_virial) */
+    uint32_t reserved;
 } jmm_engine_opts;
 
 /* One per chain per sweep (the per-sweep form of the reference's per-trial
@@ -199,6 +202,15 @@ int jmm_full_energy(jmm_engine *e, double *U_out);
  * one copy.  Synchronous. */
 int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out);
 
+/* jmm_read_samples plus, for an engine created with sample_virial, the
+ * virial and hypervirial after every sweep: virial[(c * max_sweeps + s) * 2]
+ * = W, [... + 1] = HV (the reference's per-trial Vir/HV sums of
+ * updateThermo, src/jmmMCState.cpp:1941-1961, per sweep).  virial may be
+ * NULL; a non-NULL virial on an engine without sample_virial is
+ * JMM_ESTATE.  Drains the buffer as jmm_read_samples.  Synchronous. */
+int jmm_read_samples_virial(jmm_engine *e, jmm_sample *out, double *virial, uint32_t max_sweeps,
+                            uint32_t *n_out);
+
 /* Replaces printCoords (src/jmmMCState.cpp:1007-1017) and the restart read
  * (:572-765): chain c's positions (sorted), length, running energy and sweep
  * counter.  Any output pointer may be NULL.  Synchronous. */
@@ -213,6 +225,16 @@ int jmm_get_states(jmm_engine *e, double *x, double *L, double *U);
  * length and sweep counter; the running energy is recomputed in FP64.
  * Synchronous. */
 int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t sweep);
+
+/* Bulk form of jmm_set_state for every chain (restart of a whole engine, or
+ * production runs started from one equilibrated configuration): x as in
+ * jmm_get_states, L[n_chains]; U[n_chains] installs the running energies
+ * as given (a checkpoint's, for a bit-exact continuation) or, when NULL,
+ * recomputes them in FP64; sweep[n_chains] (nullable) sets the sweep
+ * counters, which key the random numbers.  Every chain is validated before
+ * any is changed.  Synchronous.  Replaces the reference's restart read
+ * (src/jmmMCState.cpp:572-765) for many chains at once. */
+int jmm_set_states(jmm_engine *e, const double *x, const double *L, const double *U, const uint64_t *sweep);
 
 /* Step sizes between sweeps, for step-size adaptation (the reference's
  * maxDisAdjust/maxDVAdjust, src/jmmMCState.cpp:2100-2139, rescale maxStep and
@@ -279,7 +301,10 @@ int jmm_chain_layout(jmm_engine *e, int chain, jmm_layout *out);
 
 /* Kernel timing of the sweep kernel, from HIP events recorded on the
  * engine's stream around every sweep-block launch since the last reset:
- * number of launches, total milliseconds, total sweeps.  Synchronous. */
+ * number of launches, total milliseconds, total sweeps.  The engine keeps
+ * at most 64 event pairs in flight (older ones are read into the totals as
+ * the queue advances), so long runs hold a bounded number of events.
+ * Synchronous. */
 int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *sweeps, int reset);
 
 /* Observables of every chain's current configuration, for the

@@ -20,8 +20,8 @@ FORMS = {"plain_fp64": FORM_PLAIN_FP64, "batched_fp64": FORM_BATCHED_FP64,
 
 EXPORTED = [
     "jmm_abi_version", "jmm_last_error", "jmm_schedule", "jmm_create", "jmm_destroy",
-    "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples",
-    "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_perturb_energy", "jmm_get_counters",
+    "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples", "jmm_read_samples_virial",
+    "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_set_states", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
     "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read", "jmm_set_steps",
     "jmm_scaled_energy", "jmm_set_lengths", "jmm_chain_layout",
@@ -44,7 +44,7 @@ class EngineOpts(C.Structure):
         ("form", C.c_int32), ("device", C.c_int32), ("sample_capacity", C.c_uint32),
         ("block_sweeps", C.c_uint32), ("wg_threads", C.c_uint32), ("use_graph", C.c_uint32),
         ("log_accepts", C.c_uint32), ("lds_budget", C.c_uint32), ("seg_lds", C.c_uint32),
-        ("seg_lanes", C.c_uint32), ("reserved", C.c_uint32 * 2),
+        ("seg_lanes", C.c_uint32), ("sample_virial", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -104,9 +104,11 @@ def lib(path: str = LIB):
     L.jmm_synchronize.argtypes = [vp]
     L.jmm_full_energy.argtypes = [vp, dp]
     L.jmm_read_samples.argtypes = [vp, C.POINTER(Sample), C.c_uint32, u32p]
+    L.jmm_read_samples_virial.argtypes = [vp, C.POINTER(Sample), dp, C.c_uint32, u32p]
     L.jmm_get_state.argtypes = [vp, C.c_int, dp, dp, dp, u64p]
     L.jmm_get_states.argtypes = [vp, dp, dp, dp]
     L.jmm_set_state.argtypes = [vp, C.c_int, dp, C.c_double, C.c_uint64]
+    L.jmm_set_states.argtypes = [vp, dp, dp, dp, u64p]
     L.jmm_perturb_energy.argtypes = [vp, C.c_int, C.c_double]
     L.jmm_get_counters.argtypes = [vp, C.c_int, C.POINTER(Counters)]
     L.jmm_read_accept_log.argtypes = [vp, C.POINTER(C.c_uint8), C.c_uint32, u32p]

@@ -232,7 +232,9 @@ struct jmm_engine {
     Ctrl *d_ctrl = nullptr;
     double *d_sU = nullptr, *d_sL = nullptr;
     uint2 *d_sA = nullptr;
+    double *d_sW = nullptr, *d_sH = nullptr;  // virial rows (opts.sample_virial)
     SampleRec *d_packed = nullptr;  // samples in the caller's layout (first read allocates)
+    double *d_packed_v = nullptr;   // virials in the caller's layout (first read allocates)
     uint32_t *d_ext = nullptr;      // external index of each internal chain
     uint64_t *d_extoff = nullptr;   // external position offset of each internal chain
     uint8_t *d_log = nullptr;
@@ -243,8 +245,12 @@ struct jmm_engine {
     int use_graph = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    // HIP events around the sweep-block launches not yet folded into the
+    // totals below (kept at most EV_KEEP pairs: older ones are read and
+    // released as the queue advances)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    uint64_t ev_sweeps = 0;
+    uint64_t ev_sweeps = 0, ev_done = 0;
+    double ev_ms_done = 0.0;
     uint64_t sweeps_run = 0;            // sweeps launched since creation
     // chains split over several workgroups (segments)
     std::vector<jmm_layout> layout;  // external index
@@ -265,6 +271,33 @@ struct jmm_engine {
 
 namespace {
 
+constexpr size_t EV_KEEP = 64;
+
+// Folds the elapsed time of the oldest event pairs into the engine's totals
+// and releases them: every pair whose end event has completed, and, while
+// more than EV_KEEP remain, the oldest after waiting for it.
+int fold_events(jmm_engine *e, bool all)
+{
+    size_t k = 0;
+    for (; k < e->ev.size(); ++k) {
+        auto &p = e->ev[k];
+        hipError_t q = hipEventQuery(p.second);
+        if (q == hipErrorNotReady) {
+            if (!all && e->ev.size() - k <= EV_KEEP) break;
+            q = hipEventSynchronize(p.second);
+        }
+        if (q != hipSuccess) return fail(JMM_EHIP, "kernel timer event: %s", hipGetErrorString(q));
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, p.first, p.second));
+        e->ev_ms_done += t;
+        e->ev_done++;
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    e->ev.erase(e->ev.begin(), e->ev.begin() + (std::ptrdiff_t)k);
+    return 0;
+}
+
 int launch_block(jmm_engine *e, uint32_t m)
 {
     for (size_t g = 0; g < e->grp.size(); ++g) {
@@ -282,6 +315,8 @@ int launch_block(jmm_engine *e, uint32_t m)
         a.sU = e->d_sU;
         a.sL = e->d_sL;
         a.sA = e->d_sA;
+        a.sW = e->d_sW;
+        a.sH = e->d_sH;
         a.acclog = e->d_log;
         a.n_items = G.n_items;
         a.n_sweeps = m;
@@ -304,6 +339,8 @@ int launch_block(jmm_engine *e, uint32_t m)
         g.a.sU = e->d_sU;
         g.a.sL = e->d_sL;
         g.a.sA = e->d_sA;
+        g.a.sW = e->d_sW;
+        g.a.sH = e->d_sH;
         g.a.acclog = e->d_log;
         g.a.n_sweeps = m;
         g.a.n_chains = (uint32_t)e->n;
@@ -459,6 +496,9 @@ int jmm_destroy(jmm_engine *e)
     (void)hipFree(e->d_sU);
     (void)hipFree(e->d_sL);
     (void)hipFree(e->d_sA);
+    (void)hipFree(e->d_sW);
+    (void)hipFree(e->d_sH);
+    (void)hipFree(e->d_packed_v);
     (void)hipFree(e->d_log);
     (void)hipFree(e->d_obs);
     (void)hipFree(e->d_grad);
@@ -638,6 +678,10 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     if (int rc = dalloc(&e->d_sA, (size_t)e->cap * n_chains)) return bail(rc);
     if (o.log_accepts)
         if (int rc = dalloc(&e->d_log, (size_t)e->cap * e->total_N)) return bail(rc);
+    if (o.sample_virial) {
+        if (int rc = dalloc(&e->d_sW, (size_t)e->cap * n_chains)) return bail(rc);
+        if (int rc = dalloc(&e->d_sH, (size_t)e->cap * n_chains)) return bail(rc);
+    }
     e->step_cap.resize(n_chains);
     for (int k = 0; k < n_chains; ++k) e->step_cap[k] = e->hch[k].max_step;
     CREATE_CHK(hipMemcpy(e->d_chains, e->hch.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
@@ -872,9 +916,14 @@ int jmm_run_sweeps(jmm_engine *e, uint32_t n_sweeps)
         // with histograms on, blocks end on every sampling sweep
         if (e->hist_every)
             m = std::min<uint32_t>(m, e->hist_every - (uint32_t)(e->sweeps_run % e->hist_every));
+        if (e->ev.size() >= EV_KEEP)
+            if (int rc = fold_events(e, false)) return rc;
         hipEvent_t a, b;
         HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return fail(JMM_EHIP, "hipEventCreate failed");
+        }
         e->ev.push_back({a, b});
         HIPCHK(hipEventRecord(a, e->stream));
         if (e->gexec && m == e->block_sweeps) {
@@ -907,21 +956,13 @@ int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *swe
     if (!e) return fail(JMM_EINVAL, "jmm_kernel_time: null engine");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
-    double tot = 0.0;
-    for (auto &p : e->ev) {
-        float t = 0.0f;
-        HIPCHK(hipEventElapsedTime(&t, p.first, p.second));
-        tot += t;
-    }
-    if (launches) *launches = e->ev.size();
-    if (ms) *ms = tot;
+    if (int rc = fold_events(e, true)) return rc;
+    if (launches) *launches = e->ev_done;
+    if (ms) *ms = e->ev_ms_done;
     if (sweeps) *sweeps = e->ev_sweeps;
     if (reset) {
-        for (auto &p : e->ev) {
-            hipEventDestroy(p.first);
-            hipEventDestroy(p.second);
-        }
-        e->ev.clear();
+        e->ev_done = 0;
+        e->ev_ms_done = 0.0;
         e->ev_sweeps = 0;
     }
     return 0;
@@ -948,7 +989,13 @@ static_assert(sizeof(jmm_sample) == sizeof(SampleRec) && offsetof(jmm_sample, L)
 
 int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out)
 {
+    return jmm_read_samples_virial(e, out, nullptr, max_sweeps, n_out);
+}
+
+int jmm_read_samples_virial(jmm_engine *e, jmm_sample *out, double *virial, uint32_t max_sweeps, uint32_t *n_out)
+{
     if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_samples: null argument");
+    if (virial && !e->d_sW) return fail(JMM_ESTATE, "jmm_read_samples_virial: engine created without sample_virial");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (int rc = check_segments(e)) return rc;
@@ -963,9 +1010,18 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
             if (int rc = dalloc(&e->d_packed, (size_t)e->cap * e->n)) return rc;
         HIPCHK(launch_pack_samples(e->d_sU, e->d_sL, e->d_sA, e->d_ext, (uint32_t)e->n, n, e->d_packed,
                                    e->stream));
+        if (virial) {
+            if (!e->d_packed_v)
+                if (int rc = dalloc(&e->d_packed_v, (size_t)e->cap * e->n * 2)) return rc;
+            HIPCHK(launch_pack_virials(e->d_sW, e->d_sH, e->d_ext, (uint32_t)e->n, n, e->d_packed_v, e->stream));
+        }
         HIPCHK(hipStreamSynchronize(e->stream));
         HIPCHK(hipMemcpy2D(out, (size_t)max_sweeps * sizeof(jmm_sample), e->d_packed, (size_t)n * sizeof(SampleRec),
                            (size_t)n * sizeof(SampleRec), (size_t)e->n, hipMemcpyDeviceToHost));
+        if (virial)
+            HIPCHK(hipMemcpy2D(virial, (size_t)max_sweeps * 2 * sizeof(double), e->d_packed_v,
+                               (size_t)n * 2 * sizeof(double), (size_t)n * 2 * sizeof(double), (size_t)e->n,
+                               hipMemcpyDeviceToHost));
     }
     e->write_pos = 0;
     std::vector<Ctrl> zc(std::max<size_t>(1, e->n_ctrl));
@@ -1068,6 +1124,45 @@ int jmm_set_state(jmm_engine *e, int chain, const double *x, double L, uint64_t 
     HIPCHK(hipMemcpy(e->d_x + d.x_off, x, sizeof(double) * s.N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_states + k, &st, sizeof st, hipMemcpyHostToDevice));
     return refresh_energies(e, (int)k);
+}
+
+int jmm_set_states(jmm_engine *e, const double *x, const double *L, const double *U, const uint64_t *sweep)
+{
+    if (!e || !x || !L) return fail(JMM_EINVAL, "jmm_set_states: null argument");
+    HIPCHK(hipSetDevice(e->device));
+    for (int c = 0; c < e->n; ++c) {
+        const jmm_chain_spec &s = e->specs[c];
+        const DevChain &d = e->hch[e->int_of_ext[c]];
+        const bool per = s.boundary == JMM_PERIODIC;
+        const double *xc = x + e->ext_off[c];
+        if (!(L[c] > 0.0) || !std::isfinite(L[c]) || (per && !(L[c] > d.Lmin)))
+            return fail(JMM_EINVAL, "jmm_set_states: chain %d: bad length %g", c, L[c]);
+        if (U && !std::isfinite(U[c])) return fail(JMM_EINVAL, "jmm_set_states: chain %d: energy not finite", c);
+        for (uint64_t i = 0; i < s.N; ++i) {
+            if (!std::isfinite(xc[i])) return fail(JMM_EINVAL, "jmm_set_states: chain %d: position %llu not finite", c, (unsigned long long)i);
+            if (!per && std::fabs(xc[i]) > L[c] / 2.0)
+                return fail(JMM_EINVAL, "jmm_set_states: chain %d: position %llu outside the walls", c, (unsigned long long)i);
+            if (i + 1 < s.N && !(xc[i + 1] - xc[i] >= d.gmin))
+                return fail(JMM_EINVAL, "jmm_set_states: chain %d: positions %llu,%llu unsorted or inside the hard core", c,
+                            (unsigned long long)i, (unsigned long long)i + 1);
+        }
+        if (per && s.N >= 2 && !(xc[0] + L[c] - xc[s.N - 1] >= d.gmin))
+            return fail(JMM_EINVAL, "jmm_set_states: chain %d: wrap-around pair inside the hard core", c);
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<double> hx(e->total_N);
+    std::vector<DevState> st(e->n);
+    HIPCHK(hipMemcpy(st.data(), e->d_states, sizeof(DevState) * e->n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < e->n; ++k) {
+        const uint32_t c = e->ext_of_int[k];
+        std::memcpy(hx.data() + e->hch[k].x_off, x + e->ext_off[c], sizeof(double) * e->hch[k].N);
+        st[k].L = L[c];
+        st[k].U = U ? U[c] : 0.0;
+        if (sweep) st[k].sweep = sweep[c];
+    }
+    HIPCHK(hipMemcpy(e->d_x, hx.data(), sizeof(double) * e->total_N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_states, st.data(), sizeof(DevState) * e->n, hipMemcpyHostToDevice));
+    return U ? 0 : refresh_energies(e, -1);
 }
 
 int jmm_set_steps(jmm_engine *e, const double *max_step, const double *max_dv)

@@ -130,6 +130,33 @@ JMM_HD double phi64(const Pot &q, double d)
     }
 }
 
+// Virial W = -d phi'(d) and hypervirial H = -d W'(d) of one pair inside the
+// cutoff, FP64 (the reference's phi[1] and phi[6], src/pot.cpp:61-66, for
+// LJ: 48/d^12 - 24/d^6 and 576/d^12 - 144/d^6 at eps = sigma = 1).  The hard
+// rod + square well has no finite virial away from contact: zero.
+template <int KIND>
+JMM_HD void virial64(const Pot &q, double d, double &W, double &H)
+{
+    if (KIND == POT_LJ) {
+        const double d3 = d * d * d;
+        const double i6 = (1.0 / (d3 * d3)) * q.sig6;
+        const double a12 = q.e4 * (i6 * i6), a6 = q.e4 * i6;
+        W = 12.0 * a12 - 6.0 * a6;
+        H = 144.0 * a12 - 36.0 * a6;
+    } else if (KIND == POT_YUKAWA) {
+        const double E = q.p0 * (exp(-q.p1 * d) / d);
+        const double kd = q.p1 * d;
+        W = E * (kd + 1.0);
+        H = E * ((kd + 1.0) * (kd + 1.0) - kd);
+    } else if (KIND == POT_HARMONIC) {
+        W = -2.0 * d * (d - 1.0);
+        H = 4.0 * d * d - 2.0 * d;
+    } else {
+        W = 0.0;
+        H = 0.0;
+    }
+}
+
 // Inside-the-cutoff test in FP64 (the FP32 form uses it so that both forms
 // sum the same pairs).
 template <int KIND>
@@ -181,6 +208,61 @@ JMM_HD float phi32(const Pot &q, float d, bool in)
         v = q.p0f * fexp(-q.p1f * d) * frcp(d) - q.shiftf;
     }
     return in ? v : 0.0f;
+}
+
+// Per-trial constants of the FP32 box-length pass (a scaling of every
+// separation by s), rounded from FP64:
+//   LJ      c12 = 4 eps (s^-12 - 1), c6 = 4 eps (s^-6 - 1), so that
+//           phi(s d) - phi(d) = i6 (c12 i6 - c6) with i6 = (sigma/d)^6
+//   Yukawa  sm1 = s - 1, is = 1/s, kx = -kappa (s - 1): phi(s d) - phi(d) =
+//           phi(d) (expm1(kx d) - sm1) is, expm1 by its Taylor series while
+//           |kx| (cutoff + step) <= 0.1 (poly = 1), else both separations
+//           are evaluated directly
+struct BoxK {
+    float c12, c6, sm1, is, kx;
+    int poly;
+};
+
+// Change of one pair's FP32 energy when its separation goes from d0 to d1
+// (d1 = s d0 up to the FP64 roundings of the scaled positions), given the
+// FP64 inside-the-cutoff flags of both: a pair entering the cutoff adds
+// phi(d1) - shift, a pair leaving it removes phi(d0) - shift, a pair inside
+// at both changes by the scaling identity's difference.  e0 receives the
+// pair's energy at d0 (phi(d0) - shift, zero outside the cutoff), so the
+// same pass sums the configuration's energy.  The hard rod + square well is
+// handled by the callers (exact counts).
+template <int KIND>
+JMM_HD float box_term32(const Pot &q, const BoxK &bk, float f0, float f1, bool in0, bool in1, float &e0)
+{
+    float dlt, r0;
+    if (KIND == POT_LJ) {
+        const float ir2 = frcp(f0 * f0);
+        const float i6 = ir2 * ir2 * ir2 * q.sig6f;
+        dlt = i6 * fmaf(bk.c12, i6, -bk.c6);
+        r0 = fmaf(i6, fmaf(q.e4f, i6, -q.e4f), -q.shiftf);
+    } else if (KIND == POT_YUKAWA) {
+        const float e0 = q.p0f * fexp(-q.p1f * f0) * frcp(f0);
+        if (bk.poly) {
+            const float x = bk.kx * f0;
+            float p = fmaf(x, 1.0f / 720.0f, 1.0f / 120.0f);
+            p = fmaf(x, p, 1.0f / 24.0f);
+            p = fmaf(x, p, 1.0f / 6.0f);
+            p = fmaf(x, p, 0.5f);
+            p = fmaf(x, p, 1.0f);
+            const float em = x * p;
+            dlt = e0 * ((em - bk.sm1) * bk.is);
+        } else {
+            dlt = q.p0f * fexp(-q.p1f * f1) * frcp(f1) - e0;
+        }
+        r0 = e0 - q.shiftf;
+    } else {
+        const float m0 = f0 - 1.0f, m1 = f1 - 1.0f;
+        dlt = m1 * m1 - m0 * m0;
+        r0 = m0 * m0;
+    }
+    e0 = in0 ? r0 : 0.0f;
+    const float enter = in1 ? r0 : -r0;
+    return (in1 ? dlt : 0.0f) + (in0 != in1 ? enter : 0.0f);
 }
 
 // FP32 inside-the-cutoff test (strict for harmonic and the square well,

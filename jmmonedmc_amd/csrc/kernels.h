@@ -6,6 +6,7 @@
 //   states   DevState[n]       L, U, sweep counter, cumulative counters
 //   sU, sL   double[cap][n]    per-sweep samples, one row per sweep (coalesced)
 //   sA       uint2[cap][n]     accepted displacements / box-length trials
+//   sW, sH   double[cap][n]    virial and hypervirial after each sweep (optional)
 //   bundles  DevBundle[m]      work items: chains packed into one workgroup
 //   slots    DevSlot[...]      chain -> lane range and LDS offset in a bundle
 //
@@ -74,6 +75,7 @@ struct SweepArgs {
     Ctrl *ctrl;
     double *sU, *sL;
     uint2 *sA;
+    double *sW, *sH;  // nullable: per-sweep virial and hypervirial rows
     uint8_t *acclog;
     uint32_t n_items, n_sweeps, n_chains, cap;
 };
@@ -105,8 +107,9 @@ struct DevSeg {
 
 // Per-segment partial sums: [segment][sweep parity][field][wave of the
 // segment's workgroup], fields dU, box-length dU, accepts, overlaps,
-// recomputed U, final counters.
-constexpr uint32_t SEG_FIELDS = 8, SEG_WAVES = 4;
+// recomputed U, final counters, virial, hypervirial, the FP32 form's
+// configuration energy from the box-length pass.
+constexpr uint32_t SEG_FIELDS = 9, SEG_WAVES = 4;
 
 struct SegArgs {
     SweepArgs a;
@@ -149,6 +152,10 @@ struct SampleRec {
 static_assert(sizeof(SampleRec) == 24, "SampleRec must match jmm_sample");
 hipError_t launch_pack_samples(const double *sU, const double *sL, const uint2 *sA, const uint32_t *ext,
                                uint32_t n_chains, uint32_t n_sweeps, SampleRec *out, hipStream_t s);
+// Virial rows to the caller's layout: out[(ext[k] * n_sweeps + s) * 2 + {0, 1}]
+// = (W, HV) of sweep s.
+hipError_t launch_pack_virials(const double *sW, const double *sH, const uint32_t *ext, uint32_t n_chains,
+                               uint32_t n_sweeps, double *out, hipStream_t s);
 hipError_t launch_hist(const HistArgs &a, hipStream_t s);
 // tw: workgroup width (256 or 512); lpb: lanes per block (1 or 2; 512 only with 2)
 const void *segmented_kernel(int fp32, int tw, int lpb);

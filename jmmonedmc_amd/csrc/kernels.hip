@@ -128,19 +128,24 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
 
 // Box-length trial energy over the pairs whose left member is in [p0, p1),
 // for a scaling of every coordinate by sc.  Sets bad when a scaled nearest
-// pair falls inside the hard core.  FP64 form: the energy change pair by
-// pair, restating fav's trial energy (src/jmmMCState.cpp:2175-2235) as a
-// difference per pair.  FP32 form: the scaled configuration's energy
-// sum phi32(s d) alone (the caller subtracts the running energy), walked to
-// the first partner beyond the cutoff at the scaled separation.
+// pair falls inside the hard core.  The energy change pair by pair,
+// restating fav's trial energy (src/jmmMCState.cpp:2175-2235) as a
+// difference per pair, each partner walk ending at the first partner beyond
+// the cutoff at both separations: FP64 form phi64(s d) - phi64(d); FP32 form
+// box_term32 of the two separations (Kahan-compensated), or the exact
+// change in the number of pairs inside the square well.  FP32 form: u0
+// receives the same pairs' energy at the unscaled separations (FP32 pair
+// energies summed in FP64; exact well counts for the square well).
 template <typename Real, int KIND>
-__device__ __forceinline__ double scaled_delta(const Pot &q, double gmin, const double *X, int64_t N,
-                                               bool per, double L, double Lp, double sc,
+__device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, double gmin, const double *X,
+                                               int64_t N, bool per, double L, double Lp, double sc,
                                                uint32_t w, uint32_t p0, uint32_t p1, uint32_t stride,
-                                               uint32_t &bad)
+                                               uint32_t &bad, double &u0)
 {
     double part = 0.0;
     float s = 0.0f, c = 0.0f;
+    int wc = 0, n0 = 0;
+    u0 = 0.0;
     for (uint32_t p = p0; p < p1 && !bad; p += stride) {
         const double xi = X[p], xis = xi * sc;
         for (uint32_t k = 1; k <= w; ++k) {
@@ -164,8 +169,16 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, double gmin, const 
                 if (d0 > q.rc && d1 > q.rc) break;
                 part += phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             } else {
-                if (d1 > q.rc) break;
-                const float t = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
+                const bool in0 = inside64<KIND>(q, d0), in1 = inside64<KIND>(q, d1);
+                if (!(in0 || in1)) break;
+                if (KIND == POT_HRSW) {
+                    wc += (int)in1 - (int)in0;
+                    n0 += (int)in0;
+                    continue;
+                }
+                float e0;
+                const float t = box_term32<KIND>(q, bk, (float)d0, (float)d1, in0, in1, e0);
+                u0 += (double)e0;
                 const float y = t - c;
                 const float tt = s + y;
                 c = (tt - s) - y;
@@ -174,7 +187,8 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, double gmin, const 
         }
     }
     if (sizeof(Real) == 8) return part;
-    return (double)s - (double)c;
+    if (KIND == POT_HRSW) u0 = -q.p1 * (double)n0;
+    return KIND == POT_HRSW ? -q.p1 * (double)wc : (double)s - (double)c;
 }
 
 // FP64 full energy of the pairs whose left member is in [p0, p1) (ECheck's
@@ -201,6 +215,34 @@ __device__ __forceinline__ double energy_part(const Pot &q, const double *X, int
     return U;
 }
 
+// Virial and hypervirial (FP64) of the pairs whose left member is in
+// [p0, p1) (the reference's Vir and HV totals, src/jmmMCState.cpp:86-96).
+template <int KIND>
+__device__ __forceinline__ void virial_part(const Pot &q, const double *X, int64_t N, bool per, double L,
+                                            uint32_t w, uint32_t p0, uint32_t p1, uint32_t stride,
+                                            double &W, double &H)
+{
+    W = 0.0;
+    H = 0.0;
+    for (uint32_t p = p0; p < p1; p += stride) {
+        const double xi = X[p];
+        for (uint32_t k = 1; k <= w; ++k) {
+            const int64_t j = (int64_t)p + k;
+            double xj;
+            if (j < N) xj = X[j];
+            else if (per) xj = X[j - N] + L;
+            else break;
+            const double d = xj - xi;
+            if (d > q.rc) break;
+            if (!inside64<KIND>(q, d)) continue;
+            double wp, hp;
+            virial64<KIND>(q, d, wp, hp);
+            W += wp;
+            H += hp;
+        }
+    }
+}
+
 // Acceptance of a box-length trial (src/jmmMCState.cpp:2249-2255).
 __device__ __forceinline__ bool vol_accept(double dU, double dL, double lns, double P, double beta,
                                            double jac, double u)
@@ -211,6 +253,7 @@ __device__ __forceinline__ bool vol_accept(double dU, double dL, double lns, dou
 
 struct VolProp {
     double sc, Lp, lns, jac, dL, u;
+    BoxK bk;  // FP32 form: the scaling constants of the pair differences
     bool valid;
 };
 
@@ -239,6 +282,16 @@ __device__ __noinline__ VolProp vol_propose(const DevChain &ch, double L, uint64
         v.dL = v.Lp - L;
     }
     v.valid = v.Lp > 0.0 && (!(ch.flags & F_PERIODIC) || v.Lp > ch.Lmin);
+    // s^-12 - 1, s^-6 - 1 and s - 1 from expm1 of ln s: no cancellation
+    const double e4 = ch.pot.e4;
+    v.bk.c12 = (float)(e4 * expm1(-12.0 * v.lns));
+    v.bk.c6 = (float)(e4 * expm1(-6.0 * v.lns));
+    const double sm1 = expm1(v.lns);
+    v.bk.sm1 = (float)sm1;
+    v.bk.is = (float)(1.0 / v.sc);
+    const double kx = -ch.pot.p1 * sm1;
+    v.bk.kx = (float)kx;
+    v.bk.poly = fabs(kx) * (ch.pot.rc + ch.max_step) <= 0.1 ? 1 : 0;
     return v;
 }
 
@@ -305,7 +358,7 @@ __device__ __forceinline__ V wave_sum(V v)
 
 template <int T>
 struct BatchedScratch {
-    double rd[(T / 64) * 2];
+    double rd[(T / 64) * 4];
     uint32_t ri[(T / 64) * 4];
     uint32_t item;
 };
@@ -421,22 +474,25 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
         // box-length trial (fav, src/jmmMCState.cpp:2161-2293)
         VolProp vp{};
         uint32_t bad = 0;
-        double vpart = 0.0;
+        double vpart = 0.0, upart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
             if (vp.valid && act)
-                vpart = scaled_delta<Real, KIND>(q, gmin, X, N, per, L, vp.Lp, vp.sc, w, start,
-                                                 start + size, 1, bad);
+                vpart = scaled_delta<Real, KIND>(q, vp.bk, gmin, X, N, per, L, vp.Lp, vp.sc, w, start,
+                                                 start + size, 1, bad, upart);
         }
         {
-            double v[2] = {dUs, vpart};
+            double v[3] = {dUs, vpart, upart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<T, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 3, 2>(v, k2, seg, sc.rd, sc.ri);
             U = U + v[0];
-            // FP32 form: the scaled energy minus the running energy
-            vpart = sizeof(Real) == 8 ? v[1] : v[1] - U;
+            vpart = v[1];
             acc = k2[0];
             bad = k2[1];
+            // FP32 form: the box-length pass summed the configuration's FP32
+            // energy over every pair; it replaces the running sum, so FP32
+            // trial errors do not accumulate between recomputes
+            if (sizeof(Real) == 4 && npt && vp.valid && !bad) U = v[2];
         }
         acc_total += acc;
         uint32_t accv = 0;
@@ -467,11 +523,24 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
             }
         }
 
+        double Ws = 0.0, Hs = 0.0;
+        if (a.sW) {
+            double v[2] = {0.0, 0.0};
+            if (act) virial_part<KIND>(q, X, N, per, L, w, start, start + size, 1, v[0], v[1]);
+            uint32_t k1[1] = {0};
+            seg_allreduce<T, 2, 1>(v, k1, seg, sc.rd, sc.ri);
+            Ws = v[0];
+            Hs = v[1];
+        }
         if (leader) {
             const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
             a.sU[row] = U;
             a.sL[row] = L;
             a.sA[row] = make_uint2(acc, accv);
+            if (a.sW) {
+                a.sW[row] = Ws;
+                a.sH[row] = Hs;
+            }
         }
     }
 
@@ -800,12 +869,13 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 
 // Box-length trial (fav, src/jmmMCState.cpp:2161-2293), one lane's share
 // over the pairs (i, i+k) whose left member is in the lane's block, in the
-// order t = 0.., k = 1.. : FP64 form, the energy change of scaling the
-// configuration by vp.sc pair by pair; FP32 form, the scaled
-// configuration's energy sum phi32(s d) alone, the caller subtracting the
-// running energy (half the pair evaluations; particle t's walk then ends at
-// its first partner beyond the cutoff at the scaled separation).  The walk
-// over (particle, partner) is one
+// order t = 0.., k = 1.. : the energy change of scaling the configuration by
+// vp.sc pair by pair, FP64 form phi64(s d) - phi64(d), FP32 form box_term32
+// (one reciprocal per LJ pair through the scaling identity) summed with
+// Kahan compensation; each particle's walk ends at its first partner beyond
+// the cutoff at both the unscaled and the scaled separation (the FP64
+// inside-the-cutoff flags decide in both forms, so both sum the same pairs).
+// The walk over (particle, partner) is one
 // flat loop with per-lane state, so lanes at different particles advance
 // together without nested divergent loops; it adds the same terms in the
 // same order as the nested walk (particle t's walk ends at its first partner
@@ -816,8 +886,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 // the chain's last block of a periodic chain, whose partners past the end
 // are the first particles shifted by L; wrap(m) returns particle m's
 // unshifted position (scaled and shifted by Lp here).  The hard-rod +
-// square-well FP32 energy is -eps times the number of pairs inside the well,
-// counted exactly.  W2: two partners of
+// square-well FP32 energy change is -eps times the change in the number of
+// pairs inside the well, counted exactly.  W2: two partners of
 // a particle per iteration (for the common walk of one partner inside and
 // the next beyond the cutoff, one iteration per particle), same terms in
 // the same order.
@@ -825,12 +895,14 @@ template <typename Real, int KIND, bool W2 = false, typename Wrap>
 __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, double gmin,
                                            const VolProp &vp, bool on, uint32_t start,
                                            uint32_t size, uint32_t a0, uint32_t gapR, bool last,
-                                           Wrap wrap, uint32_t &bad)
+                                           Wrap wrap, uint32_t &bad, double &u0)
 {
     const double *X = c.R;
     double part = 0.0;
     float sf = 0.0f, cc = 0.0f;
     int wc = 0;  // hard rod + square well, FP32: pairs inside the well (energy -eps each)
+    int n0 = 0;  // ... and at the unscaled separations
+    u0 = 0.0;
     bool run = on && size > 0;
     uint32_t t = 0, k = 1;
     double xi = X[run ? a0 : 0u];
@@ -855,11 +927,13 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         if (valid && cross2 && last) xjs2 = wrap(t + k2 - size) * vp.sc + vp.Lp;
         const double d0 = xj - xi, d1 = xjs - xis;
         const double e0 = xj2 - xi, e1 = xjs2 - xis;
+        const bool in0 = inside64<KIND>(q, d0), in1 = inside64<KIND>(q, d1);
+        const bool jn0 = inside64<KIND>(q, e0), jn1 = inside64<KIND>(q, e1);
         const bool isbad = valid && k == 1 && d1 < gmin;
-        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : d1 > q.rc);
+        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : !(in0 || in1));
         const bool add = valid && !isbad && !stop;
         const bool valid2 = add && k < kR;
-        const bool stop2 = valid2 && (sizeof(Real) == 8 ? e0 > q.rc && e1 > q.rc : e1 > q.rc);
+        const bool stop2 = valid2 && (sizeof(Real) == 8 ? e0 > q.rc && e1 > q.rc : !(jn0 || jn1));
         const bool add2 = valid2 && !stop2;
         if (sizeof(Real) == 8) {
             const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
@@ -867,10 +941,14 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
             part = add ? part + tt : part;
             part = add2 ? part + tt2 : part;
         } else if (KIND == POT_HRSW) {
-            wc += (add && inside64<KIND>(q, d1) ? 1 : 0) + (add2 && inside64<KIND>(q, e1) ? 1 : 0);
+            wc += (add ? (int)in1 - (int)in0 : 0) + (add2 ? (int)jn1 - (int)jn0 : 0);
+            n0 += (add && in0 ? 1 : 0) + (add2 && jn0 ? 1 : 0);
         } else {
-            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
-            const float tt2 = phi32<KIND>(q, (float)e1, inside64<KIND>(q, e1));
+            float g0, g1;
+            const float tt = box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, in0, in1, g0);
+            const float tt2 = box_term32<KIND>(q, vp.bk, (float)e0, (float)e1, jn0, jn1, g1);
+            u0 += add ? (double)g0 : 0.0;
+            u0 += add2 ? (double)g1 : 0.0;
             const float y = tt - cc;
             const float z = sf + y;
             cc = add ? (z - sf) - y : cc;
@@ -906,16 +984,20 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         double xjs = xj * vp.sc;
         if (valid && cross && last) xjs = wrap(t + k - size) * vp.sc + vp.Lp;
         const double d0 = xj - xi, d1 = xjs - xis;
+        const bool in0 = inside64<KIND>(q, d0), in1 = inside64<KIND>(q, d1);
         const bool isbad = valid && k == 1 && d1 < gmin;
-        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : d1 > q.rc);
+        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : !(in0 || in1));
         const bool add = valid && !isbad && !stop;
         if (sizeof(Real) == 8) {
             const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             part = add ? part + tt : part;
         } else if (KIND == POT_HRSW) {
-            wc += add && inside64<KIND>(q, d1) ? 1 : 0;
+            wc += add ? (int)in1 - (int)in0 : 0;
+            n0 += add && in0 ? 1 : 0;
         } else {
-            const float tt = phi32<KIND>(q, (float)d1, inside64<KIND>(q, d1));
+            float g0;
+            const float tt = box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, in0, in1, g0);
+            u0 += add ? (double)g0 : 0.0;
             const float y = tt - cc;
             const float z = sf + y;
             cc = add ? (z - sf) - y : cc;
@@ -937,7 +1019,30 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     }
     }
     if (sizeof(Real) == 8) return part;
+    if (KIND == POT_HRSW) u0 = -q.p1 * (double)n0;
     return KIND == POT_HRSW ? -q.p1 * (double)wc : (double)sf - (double)cc;
+}
+
+// Virial and hypervirial (FP64) of the pairs whose left member is in the
+// lane's block, read from the LDS image (the recompute's walk).
+template <int KIND>
+__device__ __forceinline__ void virial_lane(const LdsChain &c, const Pot &q, uint32_t start, uint32_t size,
+                                            uint32_t a0, uint32_t gapR, double &W, double &H)
+{
+    for (uint32_t t = 0; t < size; ++t) {
+        const uint32_t i = start + t;
+        const double xi = c.R[a0 + t];
+        const uint32_t kR = c.per ? c.w : min(c.w, c.N - 1 - i);
+        for (uint32_t k = 1; k <= kR; ++k) {
+            const double d = c.R[a0 + t + k + (t + k >= size ? gapR : 0)] - xi;
+            if (d > q.rc) break;
+            if (!inside64<KIND>(q, d)) continue;
+            double wp, hp;
+            virial64<KIND>(q, d, wp, hp);
+            W += wp;
+            H += hp;
+        }
+    }
 }
 
 template <typename Real, int KIND, int T>
@@ -1055,22 +1160,25 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         // pair by pair over the pairs whose left member is in this block
         VolProp vp{};
         uint32_t bad = 0;
-        double vpart = 0.0;
+        double vpart = 0.0, upart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
             vpart = vol_lane<Real, KIND, sizeof(Real) == 4>(
                 c, q, gmin, vp, vp.valid && act, start, size, a0, gapR, c.per && r + 1 == c.B,
-                [&](uint32_t m) { return c.R[c.wg + m]; }, bad);
+                [&](uint32_t m) { return c.R[c.wg + m]; }, bad, upart);
         }
         {
-            double v[2] = {dUs, vpart};
+            double v[3] = {dUs, vpart, upart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<T, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 3, 2>(v, k2, seg, sc.rd, sc.ri);
             U = U + v[0];
-            // FP32 form: the scaled energy minus the running energy
-            vpart = sizeof(Real) == 8 ? v[1] : v[1] - U;
+            vpart = v[1];
             acc = k2[0];
             bad = k2[1];
+            // FP32 form: the box-length pass summed the configuration's FP32
+            // energy over every pair; it replaces the running sum, so FP32
+            // trial errors do not accumulate between recomputes
+            if (sizeof(Real) == 4 && npt && vp.valid && !bad) U = v[2];
         }
         acc_total += acc;
         uint32_t accv = 0;
@@ -1118,11 +1226,24 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
             }
         }
 
+        double Ws = 0.0, Hs = 0.0;
+        if (a.sW) {
+            double v[2] = {0.0, 0.0};
+            if (act) virial_lane<KIND>(c, q, start, size, a0, gapR, v[0], v[1]);
+            uint32_t k1[1] = {0};
+            seg_allreduce<T, 2, 1>(v, k1, seg, sc.rd, sc.ri);
+            Ws = v[0];
+            Hs = v[1];
+        }
         if (leader) {
             const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
             a.sU[row] = U;
             a.sL[row] = L;
             a.sA[row] = make_uint2(acc, accv);
+            if (a.sW) {
+                a.sW[row] = Ws;
+                a.sH[row] = Hs;
+            }
         }
     }
 
@@ -1470,22 +1591,23 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         // is in the lane's block, across segment boundaries through the halo
         VolProp vp{};
         uint32_t bad = 0;
-        double vpart = 0.0;
+        double vpart = 0.0, upart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
             vpart = vol_lane<Real, KIND, sizeof(Real) == 4>(
                 c, q, gmin, vp, vp.valid && act && own, start, size, a0, gapR,
                 per && r + 1 == Bs && s + 1 == S,
-                [&](uint32_t m) { return seg_load(&g.pub_head[(uint64_t)sg.first * W + m]); }, bad);
+                [&](uint32_t m) { return seg_load(&g.pub_head[(uint64_t)sg.first * W + m]); }, bad, upart);
         }
         const uint32_t par = (uint32_t)(sweep & 1u);
         {
-            double v[2] = {dUs, vpart};
+            double v[3] = {dUs, vpart, upart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<TW, 2, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<TW, 3, 2>(v, k2, seg, sc.rd, sc.ri);
             if (tid == 0) {
-                put(par, 0, 0, 2);
-                put(par, 1, 1, 2);
+                put(par, 0, 0, 3);
+                put(par, 1, 1, 3);
+                put(par, 8, 2, 3);
                 g.part[seg_pidx(me, par, 2, 0)] = (double)k2[0];
                 g.part[seg_pidx(me, par, 3, 0)] = (double)k2[1];
             }
@@ -1493,16 +1615,19 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         sync_all();
         gather(par, 0);
         gather(par, 1);
+        gather(par, 8);
         gather_int(par, 2, 0);
         gather_int(par, 3, 0);
         U = U + tot[0];
+        // FP32 form: the configuration's FP32 energy from the box-length
+        // pass replaces the running sum (as in the whole-chain kernel)
+        if (sizeof(Real) == 4 && npt && vp.valid && tot[3] == 0.0) U = tot[8];
         acc = (uint32_t)tot[2];
         acc_total += acc;
         uint32_t accv = 0;
         if (npt) {
             ++n_vt;
-            // FP32 form: the scaled energy minus the running energy
-            const double dUv = sizeof(Real) == 8 ? tot[1] : tot[1] - U;
+            const double dUv = tot[1];
             if (vp.valid && tot[3] == 0.0 && vol_accept(dUv, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
                 if (own)
                     for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;
@@ -1545,11 +1670,31 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
             ++n_rc;
         }
 
+        double Ws = 0.0, Hs = 0.0;
+        if (a.sW) {
+            double v[2] = {0.0, 0.0};
+            if (act && own) virial_lane<KIND>(c, q, start, size, a0, gapR, v[0], v[1]);
+            uint32_t k1[1] = {0};
+            seg_allreduce<TW, 2, 1>(v, k1, seg, sc.rd, sc.ri);
+            if (tid == 0) {
+                put(par, 6, 0, 2);
+                put(par, 7, 1, 2);
+            }
+            sync_all();
+            gather(par, 6);
+            gather(par, 7);
+            Ws = tot[6];
+            Hs = tot[7];
+        }
         if (s == 0 && tid == 0) {
             const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
             a.sU[row] = U;
             a.sL[row] = L;
             a.sA[row] = make_uint2(acc, accv);
+            if (a.sW) {
+                a.sW[row] = Ws;
+                a.sH[row] = Hs;
+            }
         }
     }
 
@@ -1710,8 +1855,9 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
             const VolProp vp = vol_propose(ch, L, sweep);
             if (vp.valid) {
                 uint32_t bad = 0;
-                double part = scaled_delta<double, KIND>(q, gmin, X, N, per, L, vp.Lp, vp.sc, w,
-                                                         lane, (uint32_t)N, 64, bad);
+                double u0;
+                double part = scaled_delta<double, KIND>(q, vp.bk, gmin, X, N, per, L, vp.Lp, vp.sc, w,
+                                                         lane, (uint32_t)N, 64, bad, u0);
                 part = wave_sum(part);
                 bad = wave_sum(bad);
                 if (!bad && vol_accept(part, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
@@ -1732,11 +1878,21 @@ __device__ void run_plain(const SweepArgs &a, uint32_t ic, double *smem, uint32_
             U = Uf;
             ++n_rc;
         }
+        double Ws = 0.0, Hs = 0.0;
+        if (a.sW) {
+            virial_part<KIND>(q, X, N, per, L, w, lane, (uint32_t)N, 64, Ws, Hs);
+            Ws = wave_sum(Ws);
+            Hs = wave_sum(Hs);
+        }
         if (lane == 0) {
             const uint64_t row = (uint64_t)(soff + sw) * a.n_chains + ic;
             a.sU[row] = U;
             a.sL[row] = L;
             a.sA[row] = make_uint2(acc, accv);
+            if (a.sW) {
+                a.sW[row] = Ws;
+                a.sH[row] = Hs;
+            }
         }
     }
     if (lane == 0) {
@@ -1933,6 +2089,38 @@ __global__ void __launch_bounds__(256) k_pack_samples(const double *sU, const do
     }
 }
 
+// Virial rows [sweep][chain] to out[(ext * ns + sweep) * 2 + {0, 1}], through
+// 32 x 32 LDS tiles as k_pack_samples.
+__global__ void __launch_bounds__(256) k_pack_virials(const double *sW, const double *sH, const uint32_t *ext,
+                                                      uint32_t nc, uint32_t ns, double *out)
+{
+    __shared__ double tW[32][33], tH[32][33];
+    const uint32_t tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;
+    const uint32_t tk = (nc + 31u) / 32u, ts = (ns + 31u) / 32u;
+    const uint64_t tiles = (uint64_t)tk * ts;
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t k0 = (uint32_t)(tile % tk) * 32u, s0 = (uint32_t)(tile / tk) * 32u;
+        for (uint32_t j = ty; j < 32u; j += 8u) {
+            const uint32_t sw = s0 + j, k = k0 + tx;
+            if (sw < ns && k < nc) {
+                const uint64_t r = (uint64_t)sw * nc + k;
+                tW[j][tx] = sW[r];
+                tH[j][tx] = sH[r];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = ty; j < 32u; j += 8u) {
+            const uint32_t k = k0 + j, sw = s0 + tx;
+            if (k < nc && sw < ns) {
+                const uint64_t o = ((uint64_t)ext[k] * ns + sw) * 2;
+                out[o] = tW[tx][j];
+                out[o + 1] = tH[tx][j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ histograms
 
 // Position folded into [-L/2, L/2) on a periodic line; unchanged with walls.
@@ -2085,6 +2273,16 @@ hipError_t launch_hist(const HistArgs &a, hipStream_t s)
 hipError_t launch_lengths(const DevState *st, double *L, uint32_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_lengths, dim3((n + 255) / 256), dim3(256), 0, s, st, L, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_virials(const double *sW, const double *sH, const uint32_t *ext, uint32_t n_chains,
+                               uint32_t n_sweeps, double *out, hipStream_t s)
+{
+    const uint64_t tiles = (uint64_t)((n_chains + 31u) / 32u) * ((n_sweeps + 31u) / 32u);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, 1u << 16);
+    if (!grid) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_virials, dim3(grid), dim3(256), 0, s, sW, sH, ext, n_chains, n_sweeps, out);
     return hipGetLastError();
 }
 

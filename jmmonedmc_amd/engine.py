@@ -80,7 +80,7 @@ class Engine:
 
     def __init__(self, chains, form="batched_fp32", device=0, sample_capacity=1024,
                  block_sweeps=100, wg_threads=0, use_graph=False, log_accepts=False,
-                 lds_budget=0, seg_lds=0, seg_lanes=0):
+                 lds_budget=0, seg_lds=0, seg_lanes=0, sample_virial=False):
         self.chains = list(chains)
         self.n = len(self.chains)
         self.form = nat.FORMS[form] if isinstance(form, str) else int(form)
@@ -94,6 +94,7 @@ class Engine:
                                                                   int(log_accepts), lds_budget)
         o.seg_lds = seg_lds
         o.seg_lanes = seg_lanes
+        o.sample_virial = int(sample_virial)
         h = C.c_void_p()
         check(lib().jmm_create(specs, self.n, C.byref(o), C.byref(h)), "jmm_create")
         self.h = h
@@ -129,6 +130,20 @@ class Engine:
               "jmm_read_samples")
         return out if n.value == m else out[:, : n.value].copy()
 
+    def read_samples_virial(self, max_sweeps: int | None = None):
+        """(samples, virials): samples as read_samples, virials a float64
+        array [chain, sweep, 2] of (W, HV) after each sweep (engines created
+        with sample_virial)."""
+        m = max_sweeps or self.cap
+        out = np.empty((self.n, m), dtype=SAMPLE_DTYPE)
+        vir = np.empty((self.n, m, 2))
+        n = C.c_uint32()
+        check(lib().jmm_read_samples_virial(self.h, out.ctypes.data_as(C.POINTER(Sample)), _dp(vir), m,
+                                            C.byref(n)), "jmm_read_samples_virial")
+        if n.value == m:
+            return out, vir
+        return out[:, : n.value].copy(), vir[:, : n.value].copy()
+
     def read_accept_log(self, max_sweeps: int | None = None):
         """List (per chain) of uint8 arrays [sweep, particle]; read before read_samples."""
         m = max_sweeps or self.cap
@@ -160,6 +175,19 @@ class Engine:
     def set_state(self, chain: int, x, L: float, sweep: int = 0):
         x = np.ascontiguousarray(x, dtype=np.float64)
         check(lib().jmm_set_state(self.h, chain, _dp(x), L, sweep), "jmm_set_state")
+
+    def set_states(self, x, L, U=None, sweep=None):
+        """Every chain at once: x (list of position arrays or one flat
+        array in chain order), L[n]; U[n] installed as given, or recomputed
+        in FP64 when None; sweep[n] (None keeps the counters)."""
+        x = np.ascontiguousarray(np.concatenate(x) if isinstance(x, (list, tuple)) else x, dtype=np.float64)
+        assert x.size == int(self.Ns.sum())
+        L = np.ascontiguousarray(L, dtype=np.float64)
+        Ua = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        sw = None if sweep is None else np.ascontiguousarray(sweep, dtype=np.uint64)
+        check(lib().jmm_set_states(self.h, _dp(x), _dp(L), _dp(Ua) if Ua is not None else None,
+                                   sw.ctypes.data_as(C.POINTER(C.c_uint64)) if sw is not None else None),
+              "jmm_set_states")
 
     def set_steps(self, max_step=None, max_dv=None):
         """Per-chain displacement and box-move half-widths (None = unchanged)."""

@@ -3,10 +3,26 @@
 // Output files keep the reference's formats: config.dat.mcs frames
 // (src/jmmMCState.cpp:1009-1013) and the 13-column thermo.dat.mcs
 // (header :566-568, rows :1916-1918).  Thermo block means are over per-sweep
-// samples (the reference samples every trial); the virial and hypervirial
-// columns are evaluated where the queued sweeps are run (print and check
-// points) and held for the sweeps in between.
+// samples (the reference samples every trial), the virial and hypervirial
+// columns included: the engine computes both after every sweep.
+//
+// Steps (STEPUNIT): TRIAL, the default, counts one step per Step() call as
+// the reference does (src/jmmMCState.cpp:1734-1811), so every interval test
+// of src/Main.cpp, getStepNum() % 10000 included, fires on the steps it
+// fires on in the reference; every numTrialTypes steps queue one sweep.
+// SWEEP advances the counter by numTrialTypes per Step() (one host call per
+// sweep) and the interval tests fire when a multiple is crossed.
+//
+// Restart: every printCoords also writes checkpoint.dat.mcs, the engine
+// state of every chain in binary (positions, length, running energy, sweep
+// counter, step sizes), so a RESTART continues bit for bit where the text
+// frame alone (L printed with %.5f, chain 0 only) could not; without a
+// checkpoint the last complete config.dat.mcs frame is used as the
+// reference does.  The output files are rewound to the restart step
+// (src/jmmMCState.cpp:696-756): rows after it are removed.
 #include "jmmMCState.h"
+
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
@@ -40,6 +56,10 @@ struct MCState {
     std::vector<jmm_chain_spec> specs;
     unsigned long N = 0, numTrialTypes = 0, numSteps = 0, sn = 0, lastStep = 0;
     unsigned long pending = 0, cap = 4096, sweepsDone = 0;
+    bool trialUnit = true;           // STEPUNIT TRIAL: one step per Step()
+    bool refAdjust = false;          // ADJUST REFERENCE
+    unsigned long trialInSweep = 0;  // steps of the sweep being counted (TRIAL)
+    std::vector<double> vbuf;        // per-sweep virial and hypervirial
     double rhoSysA = 0, rhoSysSA = 0, lA = 0, lSA = 0, EA = 0, ESA = 0, lEA = 0;
     double VirA = 0, VirSA = 0, EVirA = 0, HVA = 0, HVSA = 0;
     unsigned long nSamples = 0;
@@ -121,8 +141,9 @@ static jmm_chain_spec spec_from_input(const MCInput &inp, unsigned long k, bool 
     return s;
 }
 
-// Runs the queued sweeps and folds their samples (chain 0) into the thermo
-// accumulators.
+// Runs the queued sweeps and folds their samples (chain 0), the virial and
+// hypervirial of every sweep included, into the thermo accumulators
+// (updateThermo, src/jmmMCState.cpp:1941-1961, per sweep).
 static int flush(MCState *m)
 {
     while (m->pending) {
@@ -131,17 +152,13 @@ static int flush(MCState *m)
         if (rc) return rc;
         uint32_t got = 0;
         m->buf.resize((size_t)m->cap * m->specs.size());
-        rc = jmm_read_samples(m->eng, m->buf.data(), (uint32_t)m->cap, &got);
+        m->vbuf.resize((size_t)m->cap * m->specs.size() * 2);
+        rc = jmm_read_samples_virial(m->eng, m->buf.data(), m->vbuf.data(), (uint32_t)m->cap, &got);
         if (rc) return rc;
-        double obs[6] = {0, 0, 0, 0, 0, 0};
-        std::vector<double> allobs(6 * m->specs.size());
-        rc = jmm_observables(m->eng, allobs.data(), nullptr);
-        if (rc) return rc;
-        std::copy(allobs.begin(), allobs.begin() + 6, obs);
-        m->Vir = obs[1];
-        m->HV = m->specs[0].potential == JMM_POT_LJ ? m->specs[0].pot_p[1] * obs[5] : 0.0;
         for (uint32_t s = 0; s < got; ++s) {
             const jmm_sample &x = m->buf[s];
+            m->Vir = m->vbuf[2 * s];
+            m->HV = m->vbuf[2 * s + 1];
             const double rho = (double)m->N / x.L;
             m->rhoSysA += rho;
             m->rhoSysSA += rho * rho;
@@ -163,6 +180,191 @@ static int flush(MCState *m)
         m->sweepsDone += n;
     }
     return 0;
+}
+
+// Binary checkpoint of every chain: "JMMCKPT1", then u64 chains, N, step;
+// per chain u64 sweep counter and f64 length, running energy, max_step,
+// max_dv, positions[N].  Written to a temporary file and renamed, so a
+// checkpoint on disk is always whole.
+static const char CKPT[] = "checkpoint.dat.mcs";
+
+static int write_checkpoint(MCState *m)
+{
+    const size_t nc = m->specs.size();
+    std::vector<double> x(nc * m->N), L(nc), U(nc);
+    if (int rc = jmm_get_states(m->eng, x.data(), L.data(), U.data())) return rc;
+    std::vector<jmm_counters> c(nc);
+    if (int rc = jmm_get_counters(m->eng, -1, c.data())) return rc;
+    const std::string tmp = std::string(CKPT) + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return -1;
+    const uint64_t hdr[3] = {(uint64_t)nc, (uint64_t)m->N, (uint64_t)m->sn};
+    bool ok = std::fwrite("JMMCKPT1", 1, 8, f) == 8 && std::fwrite(hdr, 8, 3, f) == 3;
+    for (size_t k = 0; k < nc && ok; ++k) {
+        const uint64_t sw = c[k].sweeps;
+        const double v[4] = {L[k], U[k], m->maxStep[k], m->maxDv[k]};
+        ok = std::fwrite(&sw, 8, 1, f) == 1 && std::fwrite(v, 8, 4, f) == 4 &&
+             std::fwrite(x.data() + k * m->N, 8, m->N, f) == m->N;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), CKPT) != 0) return -1;
+    return 0;
+}
+
+struct Checkpoint {
+    unsigned long step = 0;
+    std::vector<uint64_t> sweep;
+    std::vector<double> x, L, U, maxStep, maxDv;
+};
+
+static bool read_checkpoint(const MCState *m, Checkpoint &ck)
+{
+    FILE *f = std::fopen(CKPT, "rb");
+    if (!f) return false;
+    char magic[8];
+    uint64_t hdr[3];
+    const size_t nc = m->specs.size();
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "JMMCKPT1", 8) == 0 &&
+              std::fread(hdr, 8, 3, f) == 3 && hdr[0] == nc && hdr[1] == m->N;
+    if (ok) {
+        ck.step = (unsigned long)hdr[2];
+        ck.sweep.resize(nc);
+        ck.x.resize(nc * m->N);
+        ck.L.resize(nc);
+        ck.U.resize(nc);
+        ck.maxStep.resize(nc);
+        ck.maxDv.resize(nc);
+        for (size_t k = 0; k < nc && ok; ++k) {
+            double v[4];
+            ok = std::fread(&ck.sweep[k], 8, 1, f) == 1 && std::fread(v, 8, 4, f) == 4 &&
+                 std::fread(ck.x.data() + k * m->N, 8, m->N, f) == m->N;
+            ck.L[k] = v[0];
+            ck.U[k] = v[1];
+            ck.maxStep[k] = v[2];
+            ck.maxDv[k] = v[3];
+        }
+    }
+    std::fclose(f);
+    return ok;
+}
+
+// Removes the rows of a step-keyed output file after the restart step (the
+// reference overwrites them in place, src/jmmMCState.cpp:696-756): rows that
+// start with a step number above `step` and everything after the first such
+// row; rows without a leading number (headers) are kept.
+static void rewind_rows(const char *name, unsigned long step)
+{
+    FILE *f = std::fopen(name, "r");
+    if (!f) return;
+    char *line = nullptr;
+    size_t cap = 0;
+    long keep = 0;
+    ssize_t n;
+    while ((n = getline(&line, &cap, f)) > 0) {
+        unsigned long st;
+        if (std::sscanf(line, "%lu", &st) == 1 && st > step) break;
+        keep = std::ftell(f);
+    }
+    std::free(line);
+    std::fclose(f);
+    if (truncate(name, keep) != 0) std::printf("WARNING: could not rewind %s\n", name);
+}
+
+// The last complete config.dat.mcs frame at or before `upto` (any step when
+// upto is ULONG_MAX): its step, length and positions, and the file offset
+// just after it.  Frames are "N", "Step no.: S  Box length: L" and N
+// "i 0.0 0.0 x" rows (src/jmmMCState.cpp:1009-1013).
+static bool last_frame(const MCState *m, unsigned long upto, unsigned long &step, double &L,
+                       std::vector<double> &x, long &end)
+{
+    FILE *f = std::fopen("config.dat.mcs", "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    std::vector<double> cur;
+    while (std::fgets(line, sizeof line, f)) {
+        unsigned long n, st;
+        double Lf;
+        if (std::sscanf(line, "%lu", &n) != 1 || std::strchr(line, ':') || std::strchr(line, '.') || n != m->N)
+            continue;
+        if (!std::fgets(line, sizeof line, f)) break;
+        if (std::sscanf(line, "Step no.: %lu  Box length: %lf", &st, &Lf) != 2) continue;
+        cur.clear();
+        for (unsigned long i = 0; i < n && std::fgets(line, sizeof line, f); ++i) {
+            unsigned long pn;
+            double a, b, z;
+            if (std::sscanf(line, "%lu %lf %lf %lf", &pn, &a, &b, &z) != 4) break;
+            cur.push_back(z);
+        }
+        if (cur.size() == m->N && st <= upto) {
+            x = cur;
+            step = st;
+            L = Lf;
+            end = std::ftell(f);
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+// Restart (src/jmmMCState.cpp:572-765): the checkpoint when there is one,
+// else chain 0's last complete text frame; then the output files are cut
+// back to the restart step.
+static bool restart(MCState *m)
+{
+    Checkpoint ck;
+    unsigned long step = 0;
+    double Lf = 0.0;
+    std::vector<double> xf;
+    long end = 0;
+    const size_t nc = m->specs.size();
+    if (read_checkpoint(m, ck)) {
+        const uint64_t swk = ck.step / m->numTrialTypes;
+        if (m->trialUnit ? swk != ck.sweep[0] : false) {
+            std::printf("ERROR: checkpoint step %lu does not match its sweep counter %llu\n", ck.step,
+                        (unsigned long long)ck.sweep[0]);
+            return false;
+        }
+        if (int rc = jmm_set_steps(m->eng, ck.maxStep.data(), ck.maxDv.data())) {
+            die_on(rc, "jmm_set_steps");
+            return false;
+        }
+        m->maxStep = ck.maxStep;
+        m->maxDv = ck.maxDv;
+        if (int rc = jmm_set_states(m->eng, ck.x.data(), ck.L.data(), ck.U.data(), ck.sweep.data())) {
+            die_on(rc, "jmm_set_states");
+            return false;
+        }
+        step = ck.step;
+        std::printf("Checkpoint found for step number %lu\nBox length: %.17g\n", step, ck.L[0]);
+        if (!last_frame(m, step, step, Lf, xf, end)) end = 0;
+    } else {
+        if (nc > 1) {
+            std::printf("ERROR: restarting %zu chains needs %s (config.dat.mcs holds chain 0 only)\n", nc, CKPT);
+            return false;
+        }
+        if (!last_frame(m, ~0ul, step, Lf, xf, end)) {
+            std::printf("ERROR: no complete configuration in config.dat.mcs\n");
+            return false;
+        }
+        std::printf("Complete configuration found for step number %lu\nBox length: %.8g\n", step, Lf);
+        if (int rc = jmm_set_state(m->eng, 0, xf.data(), Lf, step / m->numTrialTypes)) {
+            die_on(rc, "jmm_set_state");
+            return false;
+        }
+    }
+    if (truncate("config.dat.mcs", end) != 0 && end > 0) std::printf("WARNING: could not rewind config.dat.mcs\n");
+    rewind_rows("thermo.dat.mcs", step);
+    rewind_rows("rho.dat.mcs", step);
+    for (int k = 0; k < m->inp.gns; ++k) {
+        char name[64];
+        std::snprintf(name, sizeof name, "g%d.dat.mcs", k);
+        rewind_rows(name, step);
+    }
+    m->sn = m->lastStep = step;
+    m->trialInSweep = m->trialUnit ? step % m->numTrialTypes : 0;
+    return true;
 }
 
 static bool crossed(const MCState *m, unsigned long interval)
@@ -194,12 +396,19 @@ struct MCState *setupMCS(struct MCInput inp)
             return nullptr;
         }
     }
+    m->trialUnit = inp.stepUnit != "SWEEP";
+    m->refAdjust = inp.adjust == "REFERENCE";
+    if (m->trialUnit && inp.ns % m->numTrialTypes)
+        std::printf("Note: NUMSTEPS %lu is not a whole number of sweeps of %lu trials; the last %lu "
+                    "steps complete no sweep and are not run.\n",
+                    inp.ns, m->numTrialTypes, inp.ns % m->numTrialTypes);
     jmm_engine_opts o{};
     o.form = inp.form == "PLAIN_FP64" ? JMM_FORM_PLAIN_FP64
            : inp.form == "BATCHED_FP32" ? JMM_FORM_BATCHED_FP32 : JMM_FORM_BATCHED_FP64;
     o.device = inp.device;
     o.sample_capacity = (uint32_t)m->cap;
     o.block_sweeps = (uint32_t)std::max(1ul, inp.sweepBlock);
+    o.sample_virial = 1;
     int rc = jmm_create(m->specs.data(), (int)m->specs.size(), &o, &m->eng);
     if (rc) {
         die_on(rc, "jmm_create");
@@ -213,6 +422,30 @@ struct MCState *setupMCS(struct MCInput inp)
         if (rc) die_on(rc, "jmm_set_steps");
     }
     printMCP(m);
+    jmm_counters c{};
+    jmm_get_counters(m->eng, 0, &c);
+    m->E = c.U;
+    m->l = c.L;
+    if (!inp.isRestart) {
+        m->cf = std::fopen("config.dat.mcs", "w");
+        m->tf = std::fopen("thermo.dat.mcs", "w");
+        if (m->tf)
+            std::fprintf(m->tf, "Step    Econf           Econf2          L       L2      LEconf          "
+                                "rho             rho2            Virial         Virial2         EconfVir        "
+                                "HV              HV2 \n");
+    } else {
+        if (!restart(m)) {
+            freeMCS(m);
+            return nullptr;
+        }
+        jmm_get_counters(m->eng, 0, &c);
+        m->E = c.U;
+        m->l = c.L;
+        m->cf = std::fopen("config.dat.mcs", "a");
+        m->tf = std::fopen("thermo.dat.mcs", "a");
+    }
+    // histograms: the first sample is the configuration the run starts from
+    // (after a restart, the restored one)
     if ((inp.rhopi && inp.rhonb && inp.rbw > 0.0) || (inp.gpi && inp.gns > 0 && inp.gnb && inp.gbw > 0.0)) {
         jmm_hist_spec hs{};
         hs.rhonb = (uint32_t)std::max(1ul, inp.rhonb);
@@ -239,67 +472,20 @@ struct MCState *setupMCS(struct MCInput inp)
                 std::snprintf(name, sizeof name, "g%u.dat.mcs", k);
                 m->gf.push_back(std::fopen(name, mode));
             }
-            jmm_hist_sample(m->eng);  // the configuration at the start
+            // the configuration at the start (a restart continues the
+            // histogram blocks of the uninterrupted run: no extra sample)
+            if (!inp.isRestart) jmm_hist_sample(m->eng);
         }
     }
-    jmm_counters c{};
-    jmm_get_counters(m->eng, 0, &c);
-    m->E = c.U;
-    m->l = c.L;
-    if (!inp.isRestart) {
-        m->cf = std::fopen("config.dat.mcs", "w");
-        m->tf = std::fopen("thermo.dat.mcs", "w");
-        if (m->tf)
-            std::fprintf(m->tf, "Step    Econf           Econf2          L       L2      LEconf          "
-                                "rho             rho2            Virial         Virial2         EconfVir        "
-                                "HV              HV2 \n");
-    } else {
-        // restart from the last complete frame of config.dat.mcs
-        FILE *f = std::fopen("config.dat.mcs", "r");
-        std::vector<double> x, best;
-        unsigned long step = 0, bestStep = 0;
-        double L = 0, bestL = 0;
-        char line[512];
-        while (f && std::fgets(line, sizeof line, f)) {
-            unsigned long n;
-            if (std::sscanf(line, "%lu", &n) == 1 && !std::strchr(line, ':') && n == m->N &&
-                std::strchr(line, '.') == nullptr) {
-                if (!std::fgets(line, sizeof line, f)) break;
-                if (std::sscanf(line, "Step no.: %lu  Box length: %lf", &step, &L) != 2) continue;
-                x.clear();
-                for (unsigned long i = 0; i < n && std::fgets(line, sizeof line, f); ++i) {
-                    unsigned long pn;
-                    double a, b, z;
-                    if (std::sscanf(line, "%lu %lf %lf %lf", &pn, &a, &b, &z) == 4) x.push_back(z);
-                }
-                if (x.size() == m->N) {
-                    best = x;
-                    bestStep = step;
-                    bestL = L;
-                }
-            }
+    // step-0 virial (src/Main.cpp:66-68 computes it with fad): W from the
+    // observables; HV = sigma dW/dsigma for LJ, the reference's only
+    // potential with a hypervirial (zero otherwise until the first sweep)
+    {
+        std::vector<double> obs(6 * m->specs.size());
+        if (jmm_observables(m->eng, obs.data(), nullptr) == 0) {
+            m->Vir = obs[1];
+            m->HV = m->specs[0].potential == JMM_POT_LJ ? m->specs[0].pot_p[1] * obs[5] : 0.0;
         }
-        if (f) std::fclose(f);
-        if (best.empty()) {
-            std::printf("ERROR: no complete configuration in config.dat.mcs\n");
-            freeMCS(m);
-            return nullptr;
-        }
-        std::printf("Complete configuration found for step number %lu\nBox length: %.8g\n", bestStep, bestL);
-        for (size_t k = 0; k < m->specs.size(); ++k) {
-            rc = jmm_set_state(m->eng, (int)k, best.data(), bestL, bestStep / m->numTrialTypes);
-            if (rc) {
-                die_on(rc, "jmm_set_state");
-                freeMCS(m);
-                return nullptr;
-            }
-        }
-        m->sn = m->lastStep = bestStep;
-        jmm_get_counters(m->eng, 0, &c);
-        m->E = c.U;
-        m->l = c.L;
-        m->cf = std::fopen("config.dat.mcs", "a");
-        m->tf = std::fopen("thermo.dat.mcs", "a");
     }
     return m;
 }
@@ -355,7 +541,7 @@ unsigned long incrementStep(struct MCState *m)
     return collective<unsigned long>([m]() -> unsigned long {
         if (m->sn >= m->numSteps) return 0;
         m->lastStep = m->sn;
-        m->sn += m->numTrialTypes;
+        m->sn += m->trialUnit ? 1 : m->numTrialTypes;
         if (m->sn / 10000 != m->lastStep / 10000) printStep(m);
         return m->sn;
     });
@@ -364,7 +550,11 @@ unsigned long incrementStep(struct MCState *m)
 int Step(struct MCState *m)
 {
     return collective<int>([m] {
-        m->pending++;
+        // TRIAL: a sweep is queued once its numTrialTypes steps are counted
+        if (!m->trialUnit || ++m->trialInSweep == m->numTrialTypes) {
+            m->trialInSweep = 0;
+            m->pending++;
+        }
         if (m->pending >= m->cap) {
             if (int rc = flush(m)) return rc;
         }
@@ -430,6 +620,8 @@ int printCoords(struct MCState *m)
     std::fprintf(m->cf, "%lu\nStep no.: %lu  Box length: %.5f\n", m->N, m->sn, L);
     for (unsigned long i = 0; i < m->N; ++i) std::fprintf(m->cf, "%lu  0.0  0.0  %.8G\n", i, x[i]);
     std::fflush(m->cf);
+    // the exact state of every chain at this frame, for a bit-exact restart
+    if (write_checkpoint(m)) std::printf("WARNING: could not write %s\n", CKPT);
     return 0;
 }
 
@@ -519,11 +711,12 @@ int ECheck(struct MCState *m)
 // Swendsen's rule (Physics Procedia 15 (2011) 81): step' = step *
 // ln(a P_ideal + b) / ln(a P + b), a = 0.672924, b = 0.0644284, P the
 // cumulative acceptance ratio.  The reference writes the denominator as
-// ln(a (P + b)) (src/jmmMCState.cpp:2105, :2128); the published form is used.
-static double swendsen(double step, double ratio)
+// ln(a (P + b)) (src/jmmMCState.cpp:2105, :2128): ADJUST REFERENCE uses its
+// form, the default ADJUST PUBLISHED the published one.
+static double swendsen(double step, double ratio, bool ref)
 {
     const double a = 0.672924, b = 0.0644284;
-    return step * std::log(a * 0.5 + b) / std::log(a * ratio + b);
+    return step * std::log(a * 0.5 + b) / (ref ? std::log(a * (ratio + b)) : std::log(a * ratio + b));
 }
 
 // src/jmmMCState.cpp:2100-2115: clamp [0.002, 0.5].
@@ -535,7 +728,7 @@ int maxDisAdjust(struct MCState *m)
     for (size_t k = 0; k < c.size(); ++k) {
         if (!c[k].trials) continue;
         const double ratio = (double)c[k].acc_d / (double)c[k].trials;
-        m->maxStep[k] = std::min(0.5, std::max(0.002, swendsen(m->maxStep[k], ratio)));
+        m->maxStep[k] = std::min(0.5, std::max(0.002, swendsen(m->maxStep[k], ratio, m->refAdjust)));
     }
     if (int rc = jmm_set_steps(m->eng, m->maxStep.data(), nullptr)) return rc;
     std::printf("Step: %lu  Updating max Step...dAcc: %lu,%lu   new maxStep: %.5G\n", m->sn,
@@ -544,9 +737,10 @@ int maxDisAdjust(struct MCState *m)
 }
 
 // src/jmmMCState.cpp:2120-2139: only after new box-length trials; clamp to
-// [0.002 N, 0.1 N] for the linear move (the reference sets 0.5 N above 0.1 N,
-// :2132-2133; the clamp is kept at 0.1 N), [0.002, 0.1] in ln L for the log
-// move.
+// [0.002 N, 0.1 N] for the linear move, or with ADJUST REFERENCE the
+// reference's rule, which sets 0.5 N when the new value exceeds 0.1 N
+// (:2132-2133); [0.002, 0.1] in ln L for the log move (which the reference
+// does not have).
 int maxDVAdjust(struct MCState *m)
 {
     if (int rc = flush(m)) return rc;
@@ -559,7 +753,8 @@ int maxDVAdjust(struct MCState *m)
     for (size_t k = 0; k < c.size(); ++k) {
         if (!c[k].vol_trials) continue;
         const double ratio = (double)c[k].acc_v / (double)c[k].vol_trials;
-        m->maxDv[k] = std::min(hi, std::max(lo, swendsen(m->maxDv[k], ratio)));
+        const double v = swendsen(m->maxDv[k], ratio, m->refAdjust);
+        m->maxDv[k] = v < lo ? lo : v > hi ? ((m->refAdjust && !lg) ? 0.5 * m->N : hi) : v;
     }
     if (int rc = jmm_set_steps(m->eng, nullptr, m->maxDv.data())) return rc;
     std::printf("Step: %lu  Updating max vol.ch....vAcc: %lu,%lu   new maxStep: %.5G\n", m->sn,
@@ -637,7 +832,8 @@ int relaxVolume(struct MCState *m)
 
 int isRelaxStep(struct MCState *m)
 {
-    // src/Main.cpp:173-176: every 10000 trials below step 1e6 when RELAX is set
+    // src/Main.cpp:173-176: every 10000 trials below step 1e6 when RELAX is
+    // set; with STEPUNIT TRIAL exactly the reference's getStepNum() % 10000 == 0
     return m->inp.relaxFlag > 0 && m->sn < 1000000 && crossed(m, 10000);
 }
 

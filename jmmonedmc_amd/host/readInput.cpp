@@ -147,6 +147,10 @@ bool readInputFile(const char *fstr, MCInput &inp)
             inp.lStart = num(t, 1);
         } else if (k == "DEVICE") {
             inp.device = (int)num(t, 1);
+        } else if (k == "STEPUNIT") {
+            inp.stepUnit = upper(t.size() > 1 ? t[1] : "");
+        } else if (k == "ADJUST") {
+            inp.adjust = upper(t.size() > 1 ? t[1] : "");
         } else {
             std::printf("Property command %s not understood.\n", k.c_str());
         }
@@ -200,7 +204,7 @@ int printInputFields(const MCInput &i)
                 i.nbn, i.numComputes);
     std::printf("boundary=%s\nvolMove=%s\nform=%s\nchains=%lu\nsweepBlock=%lu\nbcap=%lu\nlStart=%.17g\n",
                 i.boundary.c_str(), i.volMove.c_str(), i.form.c_str(), i.chains, i.sweepBlock, i.bcap, i.lStart);
-    std::printf("histEvery=%lu\n", i.histEvery);
+    std::printf("histEvery=%lu\nstepUnit=%s\nadjust=%s\n", i.histEvery, i.stepUnit.c_str(), i.adjust.c_str());
     std::printf("potParams=%.17g,%.17g,%.17g,%.17g\npotParamsSet=%d\n", i.potParams[0], i.potParams[1],
                 i.potParams[2], i.potParams[3], i.potParamsSet);
     return 0;

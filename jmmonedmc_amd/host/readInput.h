@@ -47,6 +47,8 @@ struct MCInput {
     unsigned long bcap = 256;          // BCAP: most particle blocks per chain
     double lStart = 0.0;               // LSTART: NPT initial length (0 = N, the reference's)
     int device = 0;                    // DEVICE
+    std::string stepUnit = "TRIAL";    // STEPUNIT TRIAL|SWEEP: what getStepNum counts per Step()
+    std::string adjust = "PUBLISHED";  // ADJUST PUBLISHED|REFERENCE: step-adaptation formula
 
     MCInput();
 };

@@ -96,17 +96,175 @@ def autocorrelation(x, max_lag: int | None = None):
     return acf if max_lag is None else acf[: max_lag + 1]
 
 
+def tau_int_window(x, c: float = 6.0):
+    """(tau, W): the integrated autocorrelation time tau = 1/2 +
+    sum_{t=1..W} rho(t) and the smallest window W with W >= c tau(W)
+    (Sokal's self-consistent window; the whole series if none)."""
+    rho = autocorrelation(x)
+    if len(rho) < 2:
+        return 0.5, 0
+    tau = 0.5 + np.cumsum(rho[1:])
+    W = np.arange(1, len(rho))
+    ok = np.flatnonzero(W >= c * tau)
+    k = ok[0] if ok.size else len(tau) - 1
+    return float(tau[k]), int(W[k])
+
+
 def tau_int(x, c: float = 6.0):
     """Integrated autocorrelation time tau = 1/2 + sum_{t>=1} rho(t), summed
     up to the smallest window W with W >= c tau(W) (Sokal).  The variance of
     the mean is 2 tau Var(x)/n."""
-    rho = autocorrelation(x)
-    tau = 0.5
-    for W in range(1, len(rho)):
-        tau += rho[W]
-        if W >= c * tau:
-            return float(tau)
-    return float(tau)
+    return tau_int_window(x, c)[0]
+
+
+def mean_se_tau(x, c: float = 6.0):
+    """Mean of a correlated series with its standard error
+    sqrt(2 tau Var(x) / n), the integrated autocorrelation time tau, the
+    number of independent samples n / (2 tau), and the standard error of
+    tau itself, tau sqrt(2 (2W + 1) / n) (Sokal)."""
+    x = np.asarray(x, dtype=np.float64).ravel()
+    n = len(x)
+    tau, W = tau_int_window(x, c)
+    tau = max(tau, 0.5)
+    se = math.sqrt(2.0 * tau * x.var() / n) if n > 1 else float("nan")
+    return dict(mean=float(x.mean()), se=se, tau=tau, n_eff=n / (2.0 * tau),
+                tau_se=tau * math.sqrt(2.0 * (2 * W + 1) / n) if n else float("nan"))
+
+
+OBSERVABLES = ("U", "L", "rho", "cP", "betaT", "tau_U")
+
+
+def chain_observables(U, L, N: int, P: float, T: float, nblocks: int = 20):
+    """Per-chain estimates with standard errors from one run's per-sweep
+    samples: <U>, <L>, <rho> = <N/L> (tau-corrected SEs), the fluctuation
+    properties cP and betaT (block formulas, block SEM), and tau_int of U
+    (Sokal SE).  Each entry is (value, se, n_eff); n_eff is the number of
+    independent samples of that estimate (for cP and betaT, the smaller of
+    the blocks' length over 2 tau of U and of L, times the block count)."""
+    U = np.asarray(U, dtype=np.float64)
+    L = np.asarray(L, dtype=np.float64)
+    out = {}
+    mu = mean_se_tau(U)
+    ml = mean_se_tau(L)
+    mr = mean_se_tau(N / L)
+    out["U"] = (mu["mean"], mu["se"], mu["n_eff"])
+    out["L"] = (ml["mean"], ml["se"], ml["n_eff"])
+    out["rho"] = (mr["mean"], mr["se"], mr["n_eff"])
+    b = len(U) // nblocks
+    if b >= 2:
+        f = fluctuations_from_samples(U, L, P, T, nblocks=nblocks)
+        blk = min(b / (2 * mu["tau"]), b / (2 * ml["tau"])) * nblocks
+        out["cP"] = (f["cP"][0], f["cP"][1], blk)
+        out["betaT"] = (f["betaT"][0], f["betaT"][1], blk)
+    out["tau_U"] = (mu["tau"], mu["tau_se"], mu["n_eff"])
+    return out
+
+
+def _acf_rows(X):
+    """Normalised autocorrelation of every row (biased estimator, FFT)."""
+    n = X.shape[1]
+    Y = X - X.mean(axis=1, keepdims=True)
+    F = np.fft.rfft(Y, 2 * n, axis=1)
+    A = np.fft.irfft(F * np.conj(F), axis=1)[:, :n] / n
+    c0 = A[:, :1].copy()
+    c0[c0 == 0] = 1.0
+    return A / c0
+
+
+def _tau_rows(X, c: float = 6.0):
+    """tau_int and Sokal's window per row (as tau_int_window)."""
+    rho = _acf_rows(X)
+    n = X.shape[1]
+    tau = 0.5 + np.cumsum(rho[:, 1:], axis=1)
+    W = np.arange(1, n)
+    ok = W[None, :] >= c * tau
+    k = np.where(ok.any(axis=1), ok.argmax(axis=1), n - 2)
+    rows = np.arange(X.shape[0])
+    return np.maximum(tau[rows, k], 0.5), W[k]
+
+
+def chain_observables_batch(U, L, N, P, T, nblocks: int = 20, chunk: int = 256):
+    """chain_observables for many chains at once: U, L [chain, sweep];
+    N, P, T per chain.  Returns {observable: (value[c], se[c], n_eff[c])}
+    with the same definitions."""
+    U = np.asarray(U, dtype=np.float64)
+    L = np.asarray(L, dtype=np.float64)
+    nc, n = U.shape
+    N = np.broadcast_to(np.asarray(N, dtype=np.float64), (nc,))
+    P = np.broadcast_to(np.asarray(P, dtype=np.float64), (nc,))
+    T = np.broadcast_to(np.asarray(T, dtype=np.float64), (nc,))
+    out = {k: (np.full(nc, np.nan), np.full(nc, np.nan), np.zeros(nc)) for k in OBSERVABLES}
+    b = n // nblocks
+    for lo in range(0, nc, chunk):
+        sl = slice(lo, min(nc, lo + chunk))
+        u, l = U[sl], L[sl]
+        r = N[sl, None] / l
+        taus = {}
+        for k, X in (("U", u), ("L", l), ("rho", r)):
+            tau, W = _tau_rows(X)
+            taus[k] = (tau, W)
+            out[k][0][sl] = X.mean(axis=1)
+            out[k][1][sl] = np.sqrt(2.0 * tau * X.var(axis=1) / n)
+            out[k][2][sl] = n / (2.0 * tau)
+        tu, wu = taus["U"]
+        out["tau_U"][0][sl] = tu
+        out["tau_U"][1][sl] = tu * np.sqrt(2.0 * (2 * wu + 1) / n)
+        out["tau_U"][2][sl] = n / (2.0 * tu)
+        if b >= 2:
+            def bm(x):
+                return x[:, : b * nblocks].reshape(x.shape[0], nblocks, b).mean(axis=2)
+            f = fluctuation_properties(bm(u), bm(u * u), bm(l), bm(l * l), bm(l * u),
+                                       P[sl, None], T[sl, None])
+            blk = np.minimum(b / (2 * tu), b / (2 * taus["L"][0])) * nblocks
+            for k in ("cP", "betaT"):
+                out[k][0][sl] = f[k].mean(axis=1)
+                out[k][1][sl] = f[k].std(axis=1) / math.sqrt(nblocks)
+                out[k][2][sl] = blk
+    return out
+
+
+def _as_batch(r):
+    if isinstance(r, dict):
+        return r
+    nc = len(r)
+    out = {}
+    for k in OBSERVABLES:
+        v = np.full(nc, np.nan)
+        se = np.full(nc, np.nan)
+        ne = np.zeros(nc)
+        for c, d in enumerate(r):
+            if k in d:
+                v[c], se[c], ne[c] = d[k]
+        out[k] = (v, se, ne)
+    return out
+
+
+def compare_runs(a, b, n_eff_min: float = 30.0):
+    """z-scores of run b against run a per observable and chain:
+    z = (b - a) / sqrt(se_a^2 + se_b^2), over the chains where both
+    estimates rest on at least n_eff_min independent samples.  a, b: one
+    entry per chain of chain_observables() dicts, or chain_observables_batch
+    results.  Returns {observable: summary} with the compared and excluded
+    chain counts, the fraction within 3 standard errors, the mean, standard
+    deviation and largest |z|, and the median and root-mean-square relative
+    difference |b - a| / |a|."""
+    A, B = _as_batch(a), _as_batch(b)
+    res = {}
+    for k in OBSERVABLES:
+        va, sa, na = A[k]
+        vb, sb, nb = B[k]
+        se = np.hypot(sa, sb)
+        ok = (np.minimum(na, nb) >= n_eff_min) & (se > 0) & np.isfinite(se) & np.isfinite(vb - va)
+        z = (vb[ok] - va[ok]) / se[ok]
+        nz = ok & (va != 0)
+        rd = np.abs(vb[nz] - va[nz]) / np.abs(va[nz]) if nz.any() else np.zeros(1)
+        res[k] = dict(chains_compared=int(z.size), chains_excluded=int((~ok).sum()),
+                      frac_within_3sigma=float((np.abs(z) <= 3).mean()) if z.size else None,
+                      mean_z=float(z.mean()) if z.size else None,
+                      std_z=float(z.std()) if z.size else None,
+                      max_abs_z=float(np.abs(z).max()) if z.size else None,
+                      median_rel_diff=float(np.median(rd)), rms_rel_diff=float(np.sqrt((rd ** 2).mean())))
+    return res
 
 
 def fluctuation_properties(E, E2, L, L2, LE, P: float, T: float):

@@ -170,7 +170,7 @@ static void phi_raw(const orc_spec *s, double d, double L, double o[9])
             double m = d - 1.0;
             o[0] = m * m;
             o[1] = -2.0 * d * m;
-            o[6] = 2.0 * d * d;  /* d^2 phi'' ... reported as HV */
+            o[6] = 4.0 * d * d - 2.0 * d;  /* HV = -d dW/dd */
             o[8] = 1.0;
         }
         return;
@@ -192,6 +192,7 @@ static void phi_raw(const orc_spec *s, double d, double L, double o[9])
         o[3] = -p[0] * e;                      /* dE/dkappa */
         o[4] = ed * (p[1] * d + 1.0);          /* dW/dA */
         o[5] = -p[1] * d * d * E;              /* dW/dkappa */
+        o[6] = E * ((p[1] * d + 1.0) * (p[1] * d + 1.0) - p[1] * d); /* HV = -d dW/dd */
         o[8] = 1.0;
         return;
     }
@@ -415,14 +416,89 @@ int orc_vol_accept(double dU, double dL, double ln_s, double P, double beta,
     return bf >= 1.0 || bf > u;
 }
 
+/* --------------------------------------------- single-precision error model
+ *
+ * The FP32 form evaluates pair energies in single precision from separations
+ * formed and tested against the cutoff in double precision, and sums them
+ * with compensation.  Its decision can differ from the double-precision one
+ * only where the margin of the decision lies within the error of its energy
+ * change.  Bound of one single-precision pair evaluation at a separation d
+ * inside the cutoff, u = 2^-24:
+ *     e32(d) = 64 u (|phi(d)| + |W(d)| + |shift|),
+ * the |W| = |d phi'(d)| term for the rounding of d itself, the |phi| term for
+ * the evaluation (about ten roundings, the hardware reciprocal and
+ * exponential within two ulp each), 64 a safety factor over both; the hard
+ * rod + square well is exact (well counts).  A displacement trial's bound on
+ * |ln u + beta dE| is beta (sum of e32 over the old and new separations
+ * inside the cutoff + 8 u sum |dphi|) + 8 u (1 + beta |dE|), the last term for
+ * the single-precision exponential of the Metropolis test; a box-length
+ * trial's is beta (sum of e32 over the pairs' unscaled and scaled
+ * separations + 8 u sum |dphi|) + 1e-12 (1 + |arg|), its acceptance being
+ * evaluated in double precision.  Pinned in tests/test_oracle.py against
+ * float32 evaluations of every potential. */
+
+#define ORC_U32 (1.0 / 16777216.0)
+
+static inline int inside_of(const orc_spec *s, double d)
+{
+    if (s->potential == ORC_POT_HRSW) return d < s->pot_p[2];
+    if (s->potential == ORC_POT_HARMONIC) return d < s->cutoff;
+    return d <= s->cutoff;
+}
+
+double orc_e32_pair(const orc_spec *s, double d)
+{
+    if (s->potential == ORC_POT_HRSW) return 0.0;
+    double o[9], c[9];
+    phi_raw(s, d, 0.0, o);
+    if (o[7] != 0.0) return INFINITY;
+    const double shift = shift_terms(s, c);
+    return 64.0 * ORC_U32 * (fabs(o[0]) + fabs(o[1]) + fabs(shift));
+}
+
+/* The bound on |ln u + beta dE| of displacement trial i by md (state before
+ * the move), walking the partners as trial_core does. */
+static double trial_bound32(const orc_chain *c, uint64_t i, double md, double dE)
+{
+    const orc_spec *s = &c->s;
+    const double xi = c->x[i], xn = xi + md, rc = s->cutoff;
+    double E = 0.0, T = 0.0;
+    for (int side = 0; side < 2; ++side) {
+        for (uint64_t k = 1; k <= c->w; ++k) {
+            double xj;
+            if (!nb_pos(c, side ? (int64_t)i + (int64_t)k : (int64_t)i - (int64_t)k, &xj)) break;
+            double dold = side ? xj - xi : xi - xj, dnew = side ? xj - xn : xn - xj;
+            if (dold > rc && dnew > rc) break;
+            if (inside_of(s, dold)) E += orc_e32_pair(s, dold);
+            if (inside_of(s, dnew)) E += orc_e32_pair(s, dnew);
+            T += fabs(phi_e(s, dnew, c->shift) - phi_e(s, dold, c->shift));
+        }
+    }
+    return c->beta * (E + 8.0 * ORC_U32 * T) + 8.0 * ORC_U32 * (1.0 + c->beta * fabs(dE));
+}
+
+/* What a trial decided and how close it was (see orc_flip). */
+typedef struct decision {
+    int own;        /* the oracle's decision */
+    int gated;      /* ended at the walls, the hard core or an inadmissible length */
+    double margin;  /* distance of the decision from its threshold, ln units */
+    double bound;   /* single-precision error bound of the same (only where computed) */
+} decision;
+
 /* One displacement trial of particle i by md with acceptance uniform u.
  * Restates qad2 (src/jmmMCState.cpp:1160-1464): wall rejection before any
  * energy work (:1188-1193), energy change over the partners (:1210-1348),
  * Metropolis test (:1367-1377), running totals on acceptance (:1384-1394).
  * Partners are read from the sorted neighbour window instead of the O(N^2)
  * pair tables; pairs beyond the cutoff or the NBN limit contribute zero in
- * both, so the sums agree up to the order of addition. */
-int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out)
+ * both, so the sums agree up to the order of addition.
+ *
+ * force < 0: the oracle's own decision.  force = 0 or 1: that decision is
+ * taken instead (replaying another form's chain); *dec (nullable) receives
+ * the oracle's own decision, its margin |ln u + beta dE|, and, where the
+ * forced decision differs from it, the single-precision error bound. */
+static int trial_core(orc_chain *c, uint64_t i, double md, double u, int force, double *dE_out,
+                      decision *dec)
 {
     const orc_spec *s = &c->s;
     double xi = c->x[i];
@@ -430,6 +506,7 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
     double rc = s->cutoff;
     c->cnt[C_TRIALS]++;
     if (dE_out) *dE_out = 0.0;
+    if (dec) { dec->own = 0; dec->gated = 1; dec->margin = INFINITY; dec->bound = 0.0; }
     if (s->boundary == ORC_WALLS && fabs(xn) > c->L / 2.0) { c->cnt[C_WALL]++; return 0; }
     if (s->N >= 2) {
         double xl, xr;
@@ -458,13 +535,25 @@ int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out
     double dE = dEL + dER;
     c->cnt[C_TERMS] += terms;
     if (dE_out) *dE_out = dE;
-    if (orc_accept(dE, c->beta, u)) {
+    const int mine = orc_accept(dE, c->beta, u);
+    if (dec) {
+        dec->own = mine;
+        dec->gated = 0;
+        dec->margin = fabs(log(u) + c->beta * dE);
+        if (force >= 0 && force != mine) dec->bound = trial_bound32(c, i, md, dE);
+    }
+    if (force < 0 ? mine : force) {
         c->x[i] = xn;
         c->U += dE;
         c->cnt[C_ACC_D]++;
         return 1;
     }
     return 0;
+}
+
+int orc_trial_with(orc_chain *c, uint64_t i, double md, double u, double *dE_out)
+{
+    return trial_core(c, i, md, u, -1, dE_out, NULL);
 }
 
 /* Position t of particle i in its block (blocks of q+1 particles first,
@@ -537,16 +626,15 @@ static int scaled_dU(const orc_chain *c, double sc, double Lp, double *dU)
     return 1;
 }
 
-/* Box-length trial.  Restates fav (src/jmmMCState.cpp:2161-2293): scale all
- * coordinates by s = L'/L (:2175-2180), full energy at the scaled positions
- * (:2193-2235), acceptance with the N ln s Jacobian (:2249-2255).  The
- * log-length form draws ln s uniformly and uses an (N+1) ln s Jacobian. */
-int orc_volume_trial_with(orc_chain *c, double dv, double u, double *dU_out)
+/* The proposal of a box-length trial: scale factor, new length, ln s, the
+ * Jacobian factor and the length change; returns 0 when the new length is
+ * not admissible (non-positive, or a periodic length at or below
+ * 2 (cutoff + max_step)). */
+static int vol_proposal(const orc_chain *c, double dv, double *sc_o, double *Lp_o, double *lns_o,
+                        double *jac_o, double *dL_o)
 {
     const orc_spec *s = &c->s;
     double sc, Lp, lns, jac, dL;
-    c->cnt[C_VTRIALS]++;
-    if (dU_out) *dU_out = 0.0;
     if (s->vol_move == ORC_VOL_LINEAR) {
         Lp = c->L + dv;
         sc = Lp / c->L;
@@ -560,17 +648,75 @@ int orc_volume_trial_with(orc_chain *c, double dv, double u, double *dU_out)
         jac = (double)s->N + 1.0;
         dL = Lp - c->L;
     }
+    *sc_o = sc; *Lp_o = Lp; *lns_o = lns; *jac_o = jac; *dL_o = dL;
     if (!(Lp > 0.0)) return 0;
     if (s->boundary == ORC_PERIODIC && !(Lp > 2.0 * (s->cutoff + c->step_cap))) return 0;
+    return 1;
+}
+
+/* The bound on |ln u - arg| of a box-length trial scaling by sc to Lp
+ * (state before the move), over the pairs scaled_dU walks. */
+static double vol_bound32(const orc_chain *c, double sc, double Lp, double arg)
+{
+    const orc_spec *s = &c->s;
+    int64_t N = (int64_t)s->N;
+    double E = 0.0, T = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        double xi = c->x[i], xis = xi * sc;
+        for (uint64_t k = 1; k <= c->w; ++k) {
+            int64_t j = i + (int64_t)k;
+            double xj, xjs;
+            if (j < N) { xj = c->x[j]; xjs = xj * sc; }
+            else if (s->boundary == ORC_PERIODIC) { xj = c->x[j - N] + c->L; xjs = c->x[j - N] * sc + Lp; }
+            else break;
+            double d0 = xj - xi, d1 = xjs - xis;
+            if (d0 > s->cutoff && d1 > s->cutoff) break;
+            if (inside_of(s, d0)) E += orc_e32_pair(s, d0);
+            if (inside_of(s, d1)) E += orc_e32_pair(s, d1);
+            T += fabs(phi_e(s, d1, c->shift) - phi_e(s, d0, c->shift));
+        }
+    }
+    return c->beta * (E + 8.0 * ORC_U32 * T) + 1e-12 * (1.0 + fabs(arg));
+}
+
+/* Box-length trial.  Restates fav (src/jmmMCState.cpp:2161-2293): scale all
+ * coordinates by s = L'/L (:2175-2180), full energy at the scaled positions
+ * (:2193-2235), acceptance with the N ln s Jacobian (:2249-2255).  The
+ * log-length form draws ln s uniformly and uses an (N+1) ln s Jacobian.
+ * force < 0: the oracle's own decision; force 0/1: that decision is taken
+ * instead; *dec as in trial_core, the margin |ln u - arg| with
+ * arg = -beta (dU + P dL) + jac ln s (gated: the new length is not
+ * admissible or a scaled nearest pair overlaps). */
+static int vol_core(orc_chain *c, double dv, double u, int force, double *dU_out, decision *dec)
+{
+    const orc_spec *s = &c->s;
+    double sc, Lp, lns, jac, dL;
+    c->cnt[C_VTRIALS]++;
+    if (dU_out) *dU_out = 0.0;
+    if (dec) { dec->own = 0; dec->gated = 1; dec->margin = INFINITY; dec->bound = 0.0; }
+    if (!vol_proposal(c, dv, &sc, &Lp, &lns, &jac, &dL)) return 0;
     double dU;
     if (!scaled_dU(c, sc, Lp, &dU)) return 0;
     if (dU_out) *dU_out = dU;
-    if (!orc_vol_accept(dU, dL, lns, s->P, c->beta, jac, u)) return 0;
+    const int mine = orc_vol_accept(dU, dL, lns, s->P, c->beta, jac, u);
+    if (dec) {
+        const double arg = -c->beta * (dU + s->P * dL) + jac * lns;
+        dec->own = mine;
+        dec->gated = 0;
+        dec->margin = fabs(arg - log(u));
+        if (force >= 0 && force != mine) dec->bound = vol_bound32(c, sc, Lp, arg);
+    }
+    if (!(force < 0 ? mine : force)) return 0;
     for (uint64_t i = 0; i < s->N; ++i) c->x[i] = c->x[i] * sc;
     c->L = Lp;
     c->U += dU;
     c->cnt[C_ACC_V]++;
     return 1;
+}
+
+int orc_volume_trial_with(orc_chain *c, double dv, double u, double *dU_out)
+{
+    return vol_core(c, dv, u, -1, dU_out, NULL);
 }
 
 int orc_volume_trial(orc_chain *c, uint64_t sweep, double *dU_out)
@@ -674,6 +820,33 @@ void orc_observables_x(const orc_spec *s, const double *x, double L, double *obs
     }
 }
 
+/* Virial W = sum -d phi'(d) and hypervirial HV = sum -d W'(d) over the
+ * pairs inside the cutoff (the reference's Vir and HV totals,
+ * src/jmmMCState.cpp:86-96, accumulated by updateThermo :1941-1961). */
+void orc_virials_x(const orc_spec *s, const double *x, double L, double out[2])
+{
+    uint64_t B, M, w;
+    orc_schedule(s, &B, &M, &w);
+    int64_t N = (int64_t)s->N;
+    out[0] = out[1] = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        for (uint64_t k = 1; k <= w; ++k) {
+            int64_t j = i + (int64_t)k;
+            double xj;
+            if (j < N) xj = x[j];
+            else if (s->boundary == ORC_PERIODIC) xj = x[j - N] + L;
+            else break;
+            double d = xj - x[i];
+            if (d > s->cutoff) break;
+            if (!inside_of(s, d)) continue;
+            double o[9];
+            phi_raw(s, d, L, o);
+            out[0] += o[1];
+            out[1] += o[6];
+        }
+    }
+}
+
 /* ----------------------------------------------------------- histograms */
 
 static double fold_x(double x, double L, int per)
@@ -724,11 +897,17 @@ static void block_of(uint64_t N, uint64_t B, uint64_t b, uint64_t *start, uint64
     *start = b * q + (b < r ? b : r);
 }
 
-static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *log, int allpairs);
+static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *log, int allpairs,
+                         double *virials);
 
 void orc_sweeps(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *accept_log)
 {
-    sweep_common(c, n, samples, accept_log, 0);
+    sweep_common(c, n, samples, accept_log, 0, NULL);
+}
+
+void orc_sweeps_virial(orc_chain *c, uint64_t n, orc_sample *samples, double *virials)
+{
+    sweep_common(c, n, samples, NULL, 0, virials);
 }
 
 /* All-partner energy change: the reference's O(N) loop over every other
@@ -774,7 +953,8 @@ static int trial_allpairs(orc_chain *c, uint64_t i, double md, double u)
     return 0;
 }
 
-static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *log, int allpairs)
+static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *log, int allpairs,
+                         double *virials)
 {
     const orc_spec *s = &c->s;
     uint64_t N = s->N, B = c->B;
@@ -791,7 +971,7 @@ static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t 
                 double a, u;
                 draw_trial(c, i, t, sweep, &a, &u);
                 double md = ((a - 0.5) * 2.0) * s->max_step;
-                int acc = allpairs ? trial_allpairs(c, i, md, u) : orc_trial_with(c, i, md, u, NULL);
+                int acc = allpairs ? trial_allpairs(c, i, md, u) : trial_core(c, i, md, u, -1, NULL, NULL);
                 if (log) log[it * N + i] = (uint8_t)acc;
             }
         }
@@ -809,13 +989,96 @@ static void sweep_common(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t 
             samples[it].acc_d = (uint32_t)(c->cnt[C_ACC_D] - acc0);
             samples[it].acc_v = (uint32_t)(c->cnt[C_ACC_V] - accv0);
         }
+        if (virials) orc_virials_x(s, c->x, c->L, virials + 2 * it);
         c->sweep++;
     }
 }
 
 void orc_sweeps_allpairs(orc_chain *c, uint64_t n, orc_sample *samples)
 {
-    sweep_common(c, n, samples, NULL, 1);
+    sweep_common(c, n, samples, NULL, 1, NULL);
+}
+
+/* Replay of another form's chain (the FP32 form's): the same sweeps as
+ * orc_sweeps, taking every decision from accept_log[it*N + i] (displacement
+ * trials) and vol_acc[it] (box-length trials, NPT) instead of the oracle's
+ * own.  Where the two differ the oracle records an orc_flip with its margin
+ * and the single-precision error bound of that trial (the first max_flips
+ * of them), then continues from the replayed form's decision, so both chains
+ * stay in the same state and every later trial is compared again.  Returns
+ * the number of flips. */
+uint64_t orc_sweeps_follow(orc_chain *c, uint64_t n, const uint8_t *accept_log, const uint8_t *vol_acc,
+                           orc_sample *samples, orc_flip *flips, uint64_t max_flips)
+{
+    const orc_spec *s = &c->s;
+    uint64_t N = s->N, B = c->B;
+    uint64_t smax = (N + B - 1) / B;
+    uint64_t nflip = 0;
+    for (uint64_t it = 0; it < n; ++it) {
+        uint64_t sweep = c->sweep;
+        uint64_t acc0 = c->cnt[C_ACC_D], accv0 = c->cnt[C_ACC_V];
+        for (uint64_t t = 0; t < smax; ++t) {
+            for (uint64_t b = 0; b < B; ++b) {
+                uint64_t start, size;
+                block_of(N, B, b, &start, &size);
+                if (t >= size) continue;
+                uint64_t i = start + t;
+                double a, u;
+                draw_trial(c, i, t, sweep, &a, &u);
+                double md = ((a - 0.5) * 2.0) * s->max_step;
+                const int want = accept_log[it * N + i] ? 1 : 0;
+                decision d;
+                trial_core(c, i, md, u, want, NULL, &d);
+                if (d.gated ? want : d.own != want) {
+                    if (nflip < max_flips) {
+                        orc_flip *f = &flips[nflip];
+                        f->sweep = sweep;
+                        f->index = i;
+                        f->margin = d.gated ? INFINITY : d.margin;
+                        f->bound = d.bound;
+                        f->taken = want;
+                        f->gated = d.gated;
+                    }
+                    ++nflip;
+                }
+            }
+        }
+        if (s->ensemble == ORC_NPT) {
+            double a, u;
+            draw_volume(c, sweep, &a, &u);
+            double dv = ((a - 0.5) * 2.0) * s->max_dv;
+            const int want = vol_acc[it] ? 1 : 0;
+            decision d;
+            const int took = vol_core(c, dv, u, want, NULL, &d);
+            if ((d.gated ? want : d.own != want) || took != want) {
+                if (nflip < max_flips) {
+                    orc_flip *f = &flips[nflip];
+                    f->sweep = sweep;
+                    f->index = UINT64_MAX;
+                    f->margin = d.gated ? INFINITY : d.margin;
+                    f->bound = d.bound;
+                    f->taken = want;
+                    f->gated = d.gated;
+                }
+                ++nflip;
+            }
+        }
+        if (s->eci && (sweep + 1) % s->eci == 0) {
+            double Uf = orc_full_energy(c);
+            double dr = fabs(Uf - c->U);
+            if (dr > c->max_drift) c->max_drift = dr;
+            c->U = Uf;
+            c->cnt[C_RECOMP]++;
+        }
+        if (samples) {
+            samples[it].U = c->U;
+            samples[it].L = c->L;
+            samples[it].acc_d = (uint32_t)(c->cnt[C_ACC_D] - acc0);
+            samples[it].acc_v = (uint32_t)(c->cnt[C_ACC_V] - accv0);
+        }
+        c->sweep++;
+    }
+    return nflip;
 }
 
 /* ------------------------------------------------------------- baseline */
@@ -828,7 +1091,7 @@ double orc_run_batch(const orc_spec *specs, int n_chains, uint64_t n_sweeps, int
     for (int k = 0; k < n_chains; ++k) cs[k] = orc_chain_new(&specs[k]);
     double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
-    for (int k = 0; k < n_chains; ++k) sweep_common(cs[k], n_sweeps, NULL, NULL, allpairs);
+    for (int k = 0; k < n_chains; ++k) sweep_common(cs[k], n_sweeps, NULL, NULL, allpairs, NULL);
     double dt = omp_get_wtime() - t0;
     for (int k = 0; k < n_chains; ++k) {
         if (U_out) U_out[k] = cs[k]->U;

@@ -101,7 +101,7 @@ void orc_hist_x(const orc_spec *s, const double *x, double L, uint32_t rhonb, do
 double orc_det_exp(double x);
 
 /* potentials: out[0]=E, out[1]=W=-d*dphi/dd, out[2]=dE/dtheta0, out[3]=dE/dtheta1,
- * out[4]=dW/dtheta0, out[5]=dW/dtheta1, out[6]=HV, out[7]=1 if d is inside the
+ * out[4]=dW/dtheta0, out[5]=dW/dtheta1, out[6]=HV=-d*dW/dd, out[7]=1 if d is inside the
  * hard core (infinite energy), out[8]=1 if inside the cutoff. */
 void orc_phi(const orc_spec *s, double d, double L, double out[9]);
 /* The reference's own 9-component LJ output layout (src/pot.cpp:90-100). */
@@ -131,9 +131,34 @@ int orc_vol_accept(double dU, double dL, double ln_s, double P, double beta,
 /* sweeps: writes n samples; accept_log (nullable) receives one byte per
  * displacement trial, indexed [sweep][particle], n*N bytes */
 void orc_sweeps(orc_chain *c, uint64_t n, orc_sample *samples, uint8_t *accept_log);
+/* as orc_sweeps, and after every sweep the virial and hypervirial of the
+ * configuration into virials[2*it], virials[2*it+1] (orc_virials_x) */
+void orc_sweeps_virial(orc_chain *c, uint64_t n, orc_sample *samples, double *virials);
 /* the reference's cost model: every trial sums over all N-1 partners
  * (src/jmmMCState.cpp:1213-1348); same chain, same results up to rounding */
 void orc_sweeps_allpairs(orc_chain *c, uint64_t n, orc_sample *samples);
+
+/* A decision of a replayed chain that differs from the oracle's own. */
+typedef struct orc_flip {
+    uint64_t sweep;   /* sweep counter of the chain at the trial */
+    uint64_t index;   /* particle index; UINT64_MAX for the box-length trial */
+    double margin;    /* |ln u + beta dE| (displacement) or |ln u - arg| (box length),
+                         from the oracle's double-precision energy change; INFINITY if
+                         the oracle rejected at the walls, hard core or length bound */
+    double bound;     /* single-precision error bound of that decision (same units) */
+    int32_t taken;    /* the replayed form's decision, which the oracle followed */
+    int32_t gated;    /* 1: the oracle's rejection was not an energy decision */
+} orc_flip;
+
+/* Single-precision error bound of one pair-energy evaluation at separation d
+ * inside the cutoff (see the error model in oracle.c). */
+double orc_e32_pair(const orc_spec *s, double d);
+
+/* Replays n sweeps taking every decision from accept_log ([sweep][particle],
+ * n*N bytes) and vol_acc (n bytes, NPT): records the decisions that differ
+ * from the oracle's own (the first max_flips) and returns their number. */
+uint64_t orc_sweeps_follow(orc_chain *c, uint64_t n, const uint8_t *accept_log, const uint8_t *vol_acc,
+                           orc_sample *samples, orc_flip *flips, uint64_t max_flips);
 
 /* energies */
 double orc_full_energy(const orc_chain *c);
@@ -142,6 +167,9 @@ double orc_full_energy_x(const orc_spec *s, const double *x, double L);
 double orc_brute_energy_x(const orc_spec *s, const double *x, double L);
 /* U, W, dU/dtheta[2], dW/dtheta[2] into obs[6]; dU/dx into grad (nullable) */
 void orc_observables_x(const orc_spec *s, const double *x, double L, double *obs, double *grad);
+
+/* W = sum -d phi'(d), HV = sum -d dW/dd over the pairs inside the cutoff */
+void orc_virials_x(const orc_spec *s, const double *x, double L, double out[2]);
 
 /* CPU baseline: run n_sweeps on every chain, one chain per thread */
 double orc_run_batch(const orc_spec *specs, int n_chains, uint64_t n_sweeps, int n_threads,

@@ -37,6 +37,15 @@ class Sample(C.Structure):
     _fields_ = [("U", C.c_double), ("L", C.c_double), ("acc_d", C.c_uint32), ("acc_v", C.c_uint32)]
 
 
+class Flip(C.Structure):
+    _fields_ = [("sweep", C.c_uint64), ("index", C.c_uint64), ("margin", C.c_double),
+                ("bound", C.c_double), ("taken", C.c_int32), ("gated", C.c_int32)]
+
+
+FLIP_DTYPE = np.dtype([("sweep", "u8"), ("index", "u8"), ("margin", "f8"), ("bound", "f8"),
+                       ("taken", "i4"), ("gated", "i4")])
+
+
 def build(force: bool = False) -> str:
     if force or not os.path.exists(LIB_PATH):
         subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -84,6 +93,13 @@ def lib():
         L.orc_vol_accept.argtypes = [C.c_double] * 7
         L.orc_sweeps.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(Sample), C.POINTER(C.c_uint8)]
         L.orc_sweeps_allpairs.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(Sample)]
+        L.orc_sweeps_follow.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                        C.POINTER(Sample), C.POINTER(Flip), C.c_uint64]
+        L.orc_sweeps_follow.restype = C.c_uint64
+        L.orc_sweeps_virial.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(Sample), dp]
+        L.orc_virials_x.argtypes = [C.POINTER(Spec), dp, C.c_double, dp]
+        L.orc_e32_pair.argtypes = [C.POINTER(Spec), C.c_double]
+        L.orc_e32_pair.restype = C.c_double
         L.orc_full_energy.argtypes = [C.c_void_p]
         L.orc_full_energy.restype = C.c_double
         L.orc_brute_energy.argtypes = [C.c_void_p]
@@ -204,6 +220,39 @@ class Chain:
                        dtype=[("U", "f8"), ("L", "f8"), ("acc_d", "u4"), ("acc_v", "u4")])
         return (arr, lg) if log else arr
 
+    def sweeps_virial(self, n):
+        """n sweeps; returns (samples, virials[n, 2]) with the virial and
+        hypervirial of the configuration after each sweep."""
+        smp = (Sample * max(n, 1))()
+        v = np.zeros((max(n, 1), 2))
+        lib().orc_sweeps_virial(self.h, n, smp, _dp(v))
+        arr = np.array([(s.U, s.L, s.acc_d, s.acc_v) for s in smp[:n]],
+                       dtype=[("U", "f8"), ("L", "f8"), ("acc_d", "u4"), ("acc_v", "u4")])
+        return arr, v[:n]
+
+    def sweeps_follow(self, n, accept_log, vol_acc=None, max_flips=1024):
+        """Replays n sweeps taking the decisions of another form's run:
+        accept_log [n, N] (or flat) one byte per displacement trial,
+        vol_acc [n] the box-length decisions.  Returns (samples, flips,
+        n_flips): flips a structured array (sweep, index, margin, bound,
+        taken, gated) of the first max_flips decisions that differ from the
+        oracle's own, index = 2**64-1 for a box-length trial."""
+        lg = np.ascontiguousarray(np.asarray(accept_log, dtype=np.uint8).ravel())
+        assert lg.size == n * self.N
+        va = np.zeros(max(n, 1), dtype=np.uint8) if vol_acc is None else \
+            np.ascontiguousarray(np.asarray(vol_acc, dtype=np.uint8).ravel())
+        assert va.size >= n
+        smp = (Sample * max(n, 1))()
+        fl = (Flip * max(max_flips, 1))()
+        u8p = C.POINTER(C.c_uint8)
+        nf = lib().orc_sweeps_follow(self.h, n, lg.ctypes.data_as(u8p), va.ctypes.data_as(u8p), smp, fl,
+                                     max_flips)
+        arr = np.array([(s.U, s.L, s.acc_d, s.acc_v) for s in smp[:n]],
+                       dtype=[("U", "f8"), ("L", "f8"), ("acc_d", "u4"), ("acc_v", "u4")])
+        flips = np.array([(f.sweep, f.index, f.margin, f.bound, f.taken, f.gated)
+                          for f in fl[:min(nf, max_flips)]], dtype=FLIP_DTYPE)
+        return arr, flips, int(nf)
+
     def sweeps_allpairs(self, n):
         smp = (Sample * max(n, 1))()
         lib().orc_sweeps_allpairs(self.h, n, smp)
@@ -257,6 +306,19 @@ def hist_x(spec: Spec, x, L, rhonb, rbw, gns, gsw, gnb, gbw):
     lib().orc_hist_x(C.byref(spec), _dp(x), L, rhonb, rbw, gns, gsw, gnb, gbw,
                      rho.ctypes.data_as(u64p), g.ctypes.data_as(u64p))
     return rho, g[: gns * gnb].reshape(gns, gnb)
+
+
+def virials_x(spec: Spec, x, L):
+    """(W, HV) of one configuration."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    o = np.zeros(2)
+    lib().orc_virials_x(C.byref(spec), _dp(x), L, _dp(o))
+    return o
+
+
+def e32_pair(spec: Spec, d: float) -> float:
+    """Single-precision error bound of one pair-energy evaluation at d."""
+    return lib().orc_e32_pair(C.byref(spec), d)
 
 
 def u32(w):

@@ -108,6 +108,8 @@ static void errors_mode()
     EXPECT(jmm_get_state(nullptr, 0, nullptr, &d, &d, nullptr) != 0, "state null");
     EXPECT(jmm_get_states(nullptr, nullptr, &d, &d) != 0, "states null");
     EXPECT(jmm_set_state(nullptr, 0, &d, 1.0, 0) != 0, "set state null");
+    EXPECT(jmm_set_states(nullptr, &d, &d, nullptr, nullptr) != 0, "set states null");
+    EXPECT(jmm_read_samples_virial(nullptr, &smp, &d, 1, &n) != 0, "virial samples null");
     EXPECT(jmm_get_counters(nullptr, 0, &cnt) != 0, "counters null");
     EXPECT(jmm_chain_layout(nullptr, 0, &lay) != 0, "layout null");
     EXPECT(jmm_pair_observables(nullptr, 1, nullptr, nullptr, nullptr, nullptr, 64, nullptr) != 0,
@@ -137,6 +139,7 @@ static void exercise(int form, uint32_t seg_lds)
     o.block_sweeps = 8;
     o.log_accepts = 1;
     o.seg_lds = seg_lds;
+    o.sample_virial = 1;
     jmm_engine *e = nullptr;
     EXPECT(jmm_create(specs.data(), n, &o, &e) == 0 && e, "create form %d", form);
     if (!e) return;
@@ -155,8 +158,9 @@ static void exercise(int form, uint32_t seg_lds)
     uint32_t nlog = 0;
     EXPECT(jmm_read_accept_log(e, log.data(), 64, &nlog) == 0 && nlog == 20, "accept log");
     std::vector<jmm_sample> smp(n * 64);
+    std::vector<double> vir((size_t)n * 64 * 2);
     uint32_t ns = 0;
-    EXPECT(jmm_read_samples(e, smp.data(), 64, &ns) == 0 && ns == 20, "samples");
+    EXPECT(jmm_read_samples_virial(e, smp.data(), vir.data(), 64, &ns) == 0 && ns == 20, "samples");
     std::vector<double> x(total), Ls(n), Us(n);
     EXPECT(jmm_get_states(e, x.data(), Ls.data(), Us.data()) == 0, "states");
     std::vector<double> x0(specs[0].N);
@@ -165,6 +169,13 @@ static void exercise(int form, uint32_t seg_lds)
     EXPECT(jmm_get_state(e, 0, x0.data(), &L, &U, &sw) == 0 && sw == 20, "state");
     EXPECT(jmm_set_state(e, 0, x0.data(), L, sw) == 0, "set state");
     EXPECT(jmm_set_state(e, n, x0.data(), L, sw) == JMM_EINVAL, "set state range");
+    EXPECT(jmm_get_states(e, x.data(), Ls.data(), Us.data()) == 0, "states again");
+    std::vector<uint64_t> sws(n, 20);
+    EXPECT(jmm_set_states(e, x.data(), Ls.data(), Us.data(), sws.data()) == 0, "set states");
+    EXPECT(jmm_set_states(e, x.data(), Ls.data(), nullptr, nullptr) == 0, "set states, energies recomputed");
+    std::vector<double> xbad(x);
+    std::swap(xbad[0], xbad[1]);
+    EXPECT(jmm_set_states(e, xbad.data(), Ls.data(), nullptr, nullptr) == JMM_EINVAL, "set states unsorted");
     std::vector<jmm_counters> cnt(n);
     EXPECT(jmm_get_counters(e, -1, cnt.data()) == 0 && cnt[0].sweeps == 20, "counters");
     std::vector<double> Uf(n);

@@ -275,3 +275,87 @@ def test_driver_relax_volume(gpu, driver, tmp_path):
     root = brentq(lambda l: P - N * T / l + 2 * (N - 1) * (l / N - 1) / N, 20.0, 75.0)
     assert abs(lf - root) < 0.01, (lf, root)
     assert Ef == pytest.approx((N - 1) * (lf / N - 1) ** 2, rel=1e-8)
+
+
+@pytest.mark.gpu
+def test_driver_thermo_virial_columns_are_block_means(gpu, oracle, driver, tmp_path):
+    """thermo.dat.mcs's Virial, Virial2, EconfVir, HV and HV2 columns
+    (printThermo, src/jmmMCState.cpp:1916-1918) are block means of the
+    per-sweep virial and hypervirial, as the U columns are: against the
+    oracle's replay of the same sweeps (FP64 form, walls, truncated LJ)."""
+    import pyoracle
+    from jmmonedmc_amd.engine import Chain
+    from jmmonedmc_amd import _native as nat
+    from helpers import lj
+    N, blocks, per = 64, 4, 50
+    inp = tmp_path / "INPUT"
+    inp.write_text(f"ENSEMBLE NLT\nN {N}\nL 110\nT 0.9\nPOT LJcut 3.0\nMAXSTEP 0.4\n"
+                   f"NUMSTEPS {N * per * blocks}\nCPI {N * per * blocks}\nTPI {N * per}\nSEED 7\n"
+                   "BOUNDARY WALLS\nFORM BATCHED_FP64\n")
+    print("[gpu] driver thermo virial columns", flush=True)
+    r = subprocess.run([driver, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = [l.split("\t") for l in (tmp_path / "thermo.dat.mcs").read_text().splitlines()[2:]]
+    assert len(rows) == blocks
+    ch = Chain(N=N, L0=110.0, T=0.9, seed=7, boundary=nat.WALLS, max_step=0.4, **lj(shifted=0))
+    orc = pyoracle.Chain(oracle_spec(pyoracle, ch))
+    smp, vir = orc.sweeps_virial(per * blocks)
+    for b, row in enumerate(rows):
+        sl = slice(b * per, (b + 1) * per)
+        U, W, H = smp["U"][sl], vir[sl, 0], vir[sl, 1]
+        want = [U.mean(), (U * U).mean(), W.mean(), (W * W).mean(), (U * W).mean(), H.mean(), (H * H).mean()]
+        got = [float(row[k]) for k in (1, 2, 8, 9, 10, 11, 12)]
+        assert int(row[0]) == N * per * (b + 1)
+        for g, w in zip(got, want):
+            assert g == pytest.approx(w, rel=2e-7, abs=1e-9), (b, got, want)
+
+
+def _run(driver, d, text, timeout=180):
+    (d / "INPUT").write_text(text)
+    r = subprocess.run([driver, str(d / "INPUT")], cwd=d, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["BATCHED_FP64", "BATCHED_FP32"])
+def test_driver_restart_continues_bit_for_bit(gpu, driver, tmp_path, form):
+    """RESTART (src/jmmMCState.cpp:572-765) from the binary checkpoint
+    written with every config frame: a run stopped past its last frame and
+    restarted ends with the same config.dat.mcs, thermo.dat.mcs and
+    rho.dat.mcs, byte for byte, as the run that was never stopped; the rows
+    the stopped run wrote after its last frame are removed on restart, and
+    every chain of a multi-chain run is restored."""
+    N, K = 40, 200  # sweeps to the checkpoint
+    nt = N + 1
+    base = (f"ENSEMBLE NPT\nN {N}\nP 0.6\nT 1.0\nPOT LJshift 3.0\nMAXSTEP 0.5\nMAXDV 0.02\n"
+            f"CPI {nt * K}\nTPI {nt * K // 4}\nRHOPI {nt * K // 4}\nRBW 4.0\nRHONB 40\nSEED 13\n"
+            f"ENGCHECK {nt * 50}\nBOUNDARY PERIODIC\nVOLMOVE LOG\nLSTART 80\nFORM {form}\nCHAINS 3\n")
+    a = tmp_path / "a"
+    b = tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    print(f"[gpu] driver restart {form}", flush=True)
+    _run(driver, a, base + f"NUMSTEPS {nt * 2 * K}\n")
+    _run(driver, b, base + f"NUMSTEPS {nt * (K + K // 2)}\n")
+    stopped = (b / "thermo.dat.mcs").read_text()
+    assert f"\n{nt * (K + K // 4)}\t" in stopped          # rows past the checkpoint exist ...
+    out = _run(driver, b, base + f"NUMSTEPS {nt * 2 * K}\nRESTART TRUE\n")
+    assert "Checkpoint found for step number" in out
+    for name in ("config.dat.mcs", "thermo.dat.mcs", "rho.dat.mcs"):
+        assert (a / name).read_text() == (b / name).read_text(), name   # ... and are rewritten
+
+
+@pytest.mark.gpu
+def test_driver_relax_trigger_counts_trials(gpu, driver, tmp_path):
+    """With the default STEPUNIT TRIAL a step is one trial, as in the
+    reference, so src/Main.cpp:173's getStepNum() % 10000 == 0 relax
+    trigger fires at steps 10000, 20000 and 30000 of a 24-trial-per-sweep
+    chain (whose sweeps end on no multiple of 10000 until step 30000)."""
+    N = 23
+    out = _run(driver, tmp_path, f"ENSEMBLE NPT\nN {N}\nP 1.0\nT 0.3\nPOT HARMONIC\nNBN 1\nMAXSTEP 0.1\n"
+                                 f"MAXDV 0.5\nNUMSTEPS 30000\nCPI 30000\nTPI 24000\nSEED 2\nRELAX\n"
+                                 "BOUNDARY WALLS\nLSTART 35\nFORM BATCHED_FP64\n")
+    assert out.count("Relaxing the Volume") == 4, out[-3000:]
+    steps = [int(s) for s in re.findall(r"Step: (\d+)\.\.\.", out)]
+    assert steps[:4] == [0, 10000, 20000, 30000]

@@ -160,6 +160,9 @@ def test_virial_and_theta_gradients_by_finite_differences(oracle, kw, ds):
         o = oracle.phi(s, d)
         dphi = (oracle.phi(s, d + h)[0] - oracle.phi(s, d - h)[0]) / (2 * h)
         assert abs(o[1] - (-d * dphi)) <= 1e-6 * max(1, abs(o[1]))
+        # hypervirial HV = -d dW/dd
+        dW = (oracle.phi(s, d + h)[1] - oracle.phi(s, d - h)[1]) / (2 * h)
+        assert abs(o[6] - (-d * dW)) <= 1e-6 * max(1, abs(o[6])), (d, o[6], -d * dW)
         if kw["potential"] == nat.POT_HARMONIC:
             continue
         for k in (0, 1):
@@ -498,3 +501,122 @@ def test_hist_hand_binned_periodic(oracle):
     rho2, g2 = oracle.hist_x(s, [v + 3.0 for v in x], 10.0, 10, 1.0, 2, 5.0, 5, 1.0)
     assert int(rho2.sum()) == 4
     assert g2.sum(axis=0).tolist() == g.sum(axis=0).tolist()
+
+
+# ------------------------------------------------ FP32 error model and replay
+
+def _phi32_lj(d, eps=1.0, sig=1.0, shift=0.0):
+    """The kernel's FP32 LJ sequence (phi32) in numpy float32: d rounded to
+    single, 1/d^2, (1/d^2)^3 sigma^6, two fused steps (rounded separately
+    here, which only widens the error)."""
+    f = np.float32
+    d = f(d)
+    ir2 = f(1.0) / (d * d)
+    s6 = ir2 * ir2 * ir2 * f(sig ** 6)
+    e4 = f(4.0 * eps)
+    return s6 * (e4 * s6 - e4) - f(shift)
+
+
+def _phi32_yukawa(d, A=1.0, k=0.2):
+    f = np.float32
+    d = f(d)
+    return f(A) * np.exp(-f(k) * d) * (f(1.0) / d)
+
+
+@pytest.mark.parametrize("kind", ["lj", "yukawa"])
+def test_fp32_error_bound_covers_single_precision_evaluation(oracle, kind):
+    """e32(d) bounds |phi32(d) - phi64(d)| for a float32 evaluation of the
+    kernel's sequence over the whole range a chain visits (hard core to
+    cutoff), with a factor of 4 to spare for the hardware reciprocal and
+    exponential (one and two ulp); and it is far below the change a dropped
+    pair term would make, so a replay with a lost term cannot pass as a
+    tie."""
+    rng = np.random.default_rng(5)
+    if kind == "lj":
+        s = oracle.make_spec(N=4, L0=100.0, **lj())
+        ds = rng.uniform(0.5, 3.0, 4000)
+        sh = 4.0 / 3.0 ** 12 - 4.0 / 3.0 ** 6
+        f32 = [float(_phi32_lj(d, shift=sh)) for d in ds]
+    else:
+        s = oracle.make_spec(N=4, L0=100.0, **yukawa())
+        ds = rng.uniform(0.5, 20.0, 4000)
+        f32 = [float(_phi32_yukawa(d)) for d in ds]
+    for d, v in zip(ds, f32):
+        exact = oracle.phi(s, d)[0]
+        e = oracle.e32_pair(s, d)
+        assert abs(v - exact) <= e / 4, (d, v, exact, e)
+        assert e < 1e-3 * max(abs(exact), 1e-3) or e < 1e-4
+
+
+def test_fp32_box_identity_within_bound(oracle):
+    """The FP32 box-length pass's LJ form, i6 (c12 i6 - c6) with c12, c6 the
+    scaled constants, against phi(s d) - phi(d) in FP64, within the bound
+    of the pair's two separations; and the Yukawa form with its Taylor
+    expm1."""
+    f = np.float32
+    rng = np.random.default_rng(6)
+    s = oracle.make_spec(N=4, L0=100.0, **lj())
+    for _ in range(2000):
+        d0 = rng.uniform(0.5, 3.0)
+        lns = rng.uniform(-0.01, 0.01)
+        sc = math.exp(lns)
+        d1 = sc * d0
+        c12, c6 = f(4.0 * math.expm1(-12 * lns)), f(4.0 * math.expm1(-6 * lns))
+        ir2 = f(1.0) / (f(d0) * f(d0))
+        i6 = ir2 * ir2 * ir2
+        got = float(i6 * (c12 * i6 - c6))
+        if d1 > 3.0:
+            continue
+        want = oracle.phi(s, d1)[0] - oracle.phi(s, d0)[0]
+        assert abs(got - want) <= (oracle.e32_pair(s, d0) + oracle.e32_pair(s, d1)) / 4
+    y = oracle.make_spec(N=4, L0=100.0, **yukawa())
+    for _ in range(2000):
+        d0 = rng.uniform(0.5, 19.5)
+        lns = rng.uniform(-0.01, 0.01)
+        sm1 = math.expm1(lns)
+        x = f(-0.2 * sm1) * f(d0)
+        p = f(1 / 120) + x * f(1 / 720)
+        for cf in (1 / 24, 1 / 6, 0.5, 1.0):
+            p = f(cf) + x * p
+        g = (x * p - f(sm1)) * f(1.0 / math.exp(lns))
+        got = float(_phi32_yukawa(d0) * g)
+        want = oracle.phi(y, math.exp(lns) * d0)[0] - oracle.phi(y, d0)[0]
+        assert abs(got - want) <= (oracle.e32_pair(y, d0) + oracle.e32_pair(y, math.exp(lns) * d0)) / 4
+
+
+def test_follow_replay_records_only_real_disagreements(oracle):
+    """orc_sweeps_follow: replaying the oracle's own decisions records no
+    flip and ends in the same state; a decision planted into the log (a
+    single flipped byte) is recorded with a margin far beyond its bound, and
+    a box-length decision planted likewise is recorded as a box flip."""
+    s = oracle.make_spec(N=120, rho=0.5, T=1.0, P=0.5, ensemble=oracle.NPT, max_dv=0.05, seed=9)
+    a = oracle.Chain(s)
+    smp, log = a.sweeps(30, log=True)
+    b = oracle.Chain(s)
+    smp2, fl, nf = b.sweeps_follow(30, log, smp["acc_v"])
+    assert nf == 0 and np.array_equal(a.state()[0], b.state()[0]) and a.state()[1] == b.state()[1]
+    np.testing.assert_array_equal(smp2["acc_d"], smp["acc_d"])
+    lg = log.reshape(30, 120).copy()
+    t = np.flatnonzero(lg[7])[0]
+    lg[7, t] = 0
+    c = oracle.Chain(s)
+    _, fl, nf = c.sweeps_follow(30, lg, smp["acc_v"])
+    assert nf >= 1 and fl[0]["sweep"] == 7 and fl[0]["index"] == t and fl[0]["margin"] > 1e3 * fl[0]["bound"]
+    va = smp["acc_v"].astype(np.uint8).copy()
+    va[3] ^= 1
+    d = oracle.Chain(s)
+    _, fl, nf = d.sweeps_follow(30, log, va)
+    assert nf >= 1 and fl[0]["sweep"] == 3 and fl[0]["index"] == np.iinfo(np.uint64).max
+    assert fl[0]["margin"] > fl[0]["bound"]
+
+
+def test_virials_of_a_configuration_by_hand(oracle):
+    """orc_virials_x sums W and HV over the pairs inside the cutoff: three
+    LJ particles at gaps 1.0 and 1.2 (the outer pair at 2.2 inside rc = 3),
+    and the reference's closed forms 48/d^12 - 24/d^6, 576/d^12 - 144/d^6."""
+    s = oracle.make_spec(N=3, L0=100.0, boundary=oracle.WALLS, **lj())
+    x = np.array([-1.1, -0.1, 1.1])
+    W, HV = oracle.virials_x(s, x, 100.0)
+    ds = (1.0, 1.2, 2.2)
+    assert W == pytest.approx(sum(48 / d ** 12 - 24 / d ** 6 for d in ds), rel=1e-13)
+    assert HV == pytest.approx(sum(576 / d ** 12 - 144 / d ** 6 for d in ds), rel=1e-13)

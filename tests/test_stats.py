@@ -154,3 +154,77 @@ def test_fluctuation_gradient_equals_reweighted_derivative(oracle):
             v.append((o[:, 1] * wgt).sum() / wgt.sum())
         fd = (v[0] - v[1]) / (2 * h)
         assert fg[k] == pytest.approx(fd, rel=1e-6, abs=1e-8), (k, fg[k], fd)
+
+
+def test_mean_se_tau_ar1_coverage():
+    """AR(1) runs of known mean: the tau-corrected standard error covers
+    the true mean at the Gaussian rate (about 99.7 % within 3 SE over many
+    runs), where the naive iid SE would not, and the Sokal SE of tau_int
+    matches the run-to-run spread of tau."""
+    a, n, runs = 0.9, 4000, 400
+    rng = np.random.default_rng(21)
+    z, taus, tse = [], [], []
+    for r in range(runs):
+        e = rng.standard_normal(n)
+        x = np.empty(n)
+        x[0] = e[0] / math.sqrt(1 - a * a)
+        for t in range(1, n):
+            x[t] = a * x[t - 1] + e[t]
+        m = stats.mean_se_tau(x)
+        z.append(m["mean"] / m["se"])
+        taus.append(m["tau"])
+        tse.append(m["tau_se"])
+    z = np.array(z)
+    assert (np.abs(z) <= 3).mean() >= 0.98
+    assert 0.85 < z.std() < 1.15
+    assert np.std(taus) == pytest.approx(np.mean(tse), rel=0.35)
+    assert np.mean(taus) == pytest.approx((1 + a) / (2 * (1 - a)), rel=0.1)
+
+
+def test_compare_runs_null_and_planted_bias():
+    """compare_runs on synthetic chains: two independent runs of the same
+    process pass (>= 99 % within 3 sigma); a run with its mean shifted by
+    5 standard errors fails; chains with too few independent samples are
+    excluded and counted."""
+    rng = np.random.default_rng(22)
+    T, P, N = 1.0, 0.5, 100
+
+    def run(shift=0.0, a=0.5, n=2000):
+        out = []
+        for c in range(300):
+            e = rng.standard_normal((2, n))
+            U = np.empty(n)
+            L = np.empty(n)
+            U[0], L[0] = e[0, 0], e[1, 0]
+            for t in range(1, n):
+                U[t] = a * U[t - 1] + e[0, t]
+                L[t] = a * L[t - 1] + e[1, t]
+            out.append(stats.chain_observables(U - 50.0 + shift, L + 200.0, N, P, T))
+        return out
+
+    r0, r1 = run(), run()
+    cmp = stats.compare_runs(r0, r1)
+    for k in ("U", "L", "rho"):
+        assert cmp[k]["frac_within_3sigma"] >= 0.99, (k, cmp[k])
+        assert cmp[k]["chains_compared"] == 300
+    se = r0[0]["U"][1]
+    bad = stats.compare_runs(r0, run(shift=5 * se * math.sqrt(2)))
+    assert bad["U"]["frac_within_3sigma"] < 0.5
+    assert bad["L"]["frac_within_3sigma"] >= 0.99
+    few = stats.compare_runs(r0, r1, n_eff_min=1e9)
+    assert few["U"]["chains_compared"] == 0 and few["U"]["chains_excluded"] == 300
+
+
+def test_chain_observables_batch_matches_single():
+    """The vectorised per-chain statistics equal the one-chain function."""
+    rng = np.random.default_rng(23)
+    nc, n = 7, 3000
+    U = np.cumsum(rng.standard_normal((nc, n)), axis=1) * 0.01 + rng.standard_normal((nc, n)) - 40
+    L = 200 + rng.standard_normal((nc, n)) + np.sin(np.arange(n) / 50.0)
+    N, P, T = np.arange(50, 57), np.linspace(0.2, 1.0, nc), np.linspace(0.6, 1.8, nc)
+    bt = stats.chain_observables_batch(U, L, N, P, T, chunk=3)
+    for c in range(nc):
+        one = stats.chain_observables(U[c], L[c], int(N[c]), P[c], T[c])
+        for k in stats.OBSERVABLES:
+            for j in range(3):
+                assert bt[k][j][c] == pytest.approx(one[k][j], rel=1e-9, abs=1e-12), (c, k, j)
