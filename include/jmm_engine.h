@@ -211,6 +211,19 @@ int jmm_read_samples(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32
 int jmm_read_samples_virial(jmm_engine *e, jmm_sample *out, double *virial, uint32_t max_sweeps,
                             uint32_t *n_out);
 
+/* jmm_read_samples without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the transposition
+ * is staged in a fixed 16 MiB device array that the library holds per GPU
+ * until the process ends, instead of the buffer jmm_read_samples allocates
+ * on first use; no hipMalloc, hipFree or allocating library call is made.
+ * At most (16 MiB - 4 bytes per chain, rounded up to 256) / 24 bytes of
+ * chain-sweeps per call (about 699,000): a larger read is refused with
+ * JMM_EINVAL and drains nothing.  Same output, layout and draining as
+ * jmm_read_samples.  All engines on one GPU share the staging array: calls
+ * that overlap on one GPU race.  Synchronous. */
+int jmm_read_samples_noalloc(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out);
+
 /* Replaces printCoords (src/jmmMCState.cpp:1007-1017) and the restart read
  * (:572-765): chain c's positions (sorted), length, running energy and sweep
  * counter.  Any output pointer may be NULL.  Synchronous. */

@@ -20,7 +20,7 @@ FORMS = {"plain_fp64": FORM_PLAIN_FP64, "batched_fp64": FORM_BATCHED_FP64,
 
 EXPORTED = [
     "jmm_abi_version", "jmm_last_error", "jmm_schedule", "jmm_create", "jmm_destroy",
-    "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples", "jmm_read_samples_virial",
+    "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples", "jmm_read_samples_virial", "jmm_read_samples_noalloc",
     "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_set_states", "jmm_perturb_energy", "jmm_get_counters",
     "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
     "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read", "jmm_set_steps",
@@ -105,6 +105,7 @@ def lib(path: str = LIB):
     L.jmm_full_energy.argtypes = [vp, dp]
     L.jmm_read_samples.argtypes = [vp, C.POINTER(Sample), C.c_uint32, u32p]
     L.jmm_read_samples_virial.argtypes = [vp, C.POINTER(Sample), dp, C.c_uint32, u32p]
+    L.jmm_read_samples_noalloc.argtypes = [vp, C.POINTER(Sample), C.c_uint32, u32p]
     L.jmm_get_state.argtypes = [vp, C.c_int, dp, dp, dp, u64p]
     L.jmm_get_states.argtypes = [vp, dp, dp, dp]
     L.jmm_set_state.argtypes = [vp, C.c_int, dp, C.c_double, C.c_uint64]

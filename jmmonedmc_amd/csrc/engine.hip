@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -20,6 +21,21 @@
 using namespace jmm;
 
 static int refresh_energies(jmm_engine *e, int only);
+
+// Staging of jmm_read_samples_noalloc: the chain-order map and the
+// transposed sample records, in one file-scope device array of
+// SCRATCH_BYTES.  Every GPU the library runs on holds its own copy until
+// the process ends; all calls on one GPU share it (calls that overlap on
+// one GPU race).  In host code the array's name is only a stand-in for the
+// symbol: its device address is taken per device with hipGetSymbolAddress
+// at engine creation (that device current) and used from then on.
+constexpr size_t SCRATCH_BYTES = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // device address of `scratch`, by device ordinal
+}  // namespace
 
 namespace {
 
@@ -195,6 +211,23 @@ DevChain make_chain(const jmm_chain_spec &s)
     return c;
 }
 
+// The device address of `scratch` on the current device, taken once per
+// device.
+int scratch_address(int device, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return fail(JMM_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(err));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
 template <typename T>
 int dalloc(T **p, size_t n)
 {
@@ -235,6 +268,7 @@ struct jmm_engine {
     double *d_sW = nullptr, *d_sH = nullptr;  // virial rows (opts.sample_virial)
     SampleRec *d_packed = nullptr;  // samples in the caller's layout (first read allocates)
     double *d_packed_v = nullptr;   // virials in the caller's layout (first read allocates)
+    unsigned char *scratch = nullptr;  // device address of this device's `scratch` array
     uint32_t *d_ext = nullptr;      // external index of each internal chain
     uint64_t *d_extoff = nullptr;   // external position offset of each internal chain
     uint8_t *d_log = nullptr;
@@ -541,8 +575,11 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     HIPCHK(hipGetDeviceCount(&ndev));
     if (o.device < 0 || o.device >= ndev) return fail(JMM_EINVAL, "jmm_create: device %d of %d", o.device, ndev);
     HIPCHK(hipSetDevice(o.device));
+    void *scr = nullptr;
+    if (int rc = scratch_address(o.device, &scr)) return rc;
 
     jmm_engine *e = new jmm_engine();
+    e->scratch = (unsigned char *)scr;
     e->n = n_chains;
     e->form = o.form;
     e->device = o.device;
@@ -1026,6 +1063,41 @@ int jmm_read_samples_virial(jmm_engine *e, jmm_sample *out, double *virial, uint
     e->write_pos = 0;
     std::vector<Ctrl> zc(std::max<size_t>(1, e->n_ctrl));
     HIPCHK(hipMemcpy(e->d_ctrl, zc.data(), sizeof(Ctrl) * zc.size(), hipMemcpyHostToDevice));
+    *n_out = n;
+    return 0;
+}
+
+int jmm_read_samples_noalloc(jmm_engine *e, jmm_sample *out, uint32_t max_sweeps, uint32_t *n_out)
+{
+    if (!e || !n_out) return fail(JMM_EINVAL, "jmm_read_samples_noalloc: null argument");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (int rc = check_segments(e)) return rc;
+    const uint32_t n = e->write_pos;
+    if (n > max_sweeps)
+        return fail(JMM_EINVAL, "jmm_read_samples_noalloc: %u sweeps pending, room for %u", n, max_sweeps);
+    if (n && !out) return fail(JMM_EINVAL, "jmm_read_samples_noalloc: out is NULL");
+    // staging in `scratch`: the chain-order map, then the records
+    const size_t map_bytes = ((sizeof(uint32_t) * (size_t)e->n + 255) / 256) * 256;
+    const size_t rec_bytes = sizeof(SampleRec) * (size_t)n * (size_t)e->n;
+    if (map_bytes + rec_bytes > SCRATCH_BYTES)
+        return fail(JMM_EINVAL,
+                    "jmm_read_samples_noalloc: %u sweeps of %d chains need %zu bytes of staging, more than "
+                    "the %zu it holds; read after fewer sweeps or use jmm_read_samples",
+                    n, e->n, map_bytes + rec_bytes, SCRATCH_BYTES);
+    if (n) {
+        uint32_t *d_map = (uint32_t *)e->scratch;
+        SampleRec *d_rec = (SampleRec *)(e->scratch + map_bytes);
+        HIPCHK(hipMemcpyAsync(d_map, e->ext_of_int.data(), sizeof(uint32_t) * e->n, hipMemcpyHostToDevice,
+                              e->stream));
+        HIPCHK(launch_pack_samples(e->d_sU, e->d_sL, e->d_sA, d_map, (uint32_t)e->n, n, d_rec, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy2D(out, (size_t)max_sweeps * sizeof(jmm_sample), d_rec, (size_t)n * sizeof(SampleRec),
+                           (size_t)n * sizeof(SampleRec), (size_t)e->n, hipMemcpyDeviceToHost));
+    }
+    e->write_pos = 0;
+    HIPCHK(hipMemsetAsync(e->d_ctrl, 0, sizeof(Ctrl) * std::max<size_t>(1, e->n_ctrl), e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     *n_out = n;
     return 0;
 }
