@@ -156,6 +156,8 @@ Pot make_pot(const jmm_chain_spec &s)
     q.shiftf = (float)q.shift;
     q.e4f = (float)q.e4;
     q.sig6f = (float)q.sig6;
+    q.a12f = (float)(q.e4 * (q.sig6 * q.sig6));
+    q.a6f = (float)(q.e4 * q.sig6);
     q.p0f = (float)q.p0;
     q.p1f = (float)q.p1;
     q.p2f = (float)q.p2;

@@ -98,7 +98,9 @@ struct Pot {
     double e4;     // LJ: 4 eps
     double sig6;   // LJ: sigma^6
     double p0, p1, p2;
-    float rcf, shiftf, e4f, sig6f, p0f, p1f, p2f, pad;
+    float rcf, shiftf, e4f, sig6f, p0f, p1f, p2f;
+    float a12f, a6f;  // LJ: 4 eps sigma^12 and 4 eps sigma^6 (FP32 forms)
+    float pad;
 };
 
 // FP64 pair energy, operation for operation the oracle's sequence
@@ -195,10 +197,11 @@ JMM_HD float phi32(const Pot &q, float d, bool in)
 {
     float v;
     if (KIND == POT_LJ) {
-        // 4 eps s6 (s6 - 1) - shift as s6 (4 eps s6 - 4 eps) - shift, two FMAs
+        // 4 eps ((sigma/d)^12 - (sigma/d)^6) - shift as
+        // s6 (4 eps sigma^12 s6 - 4 eps sigma^6) - shift with s6 = d^-6, two FMAs
         const float ir2 = frcp(d * d);
-        const float s6 = ir2 * ir2 * ir2 * q.sig6f;
-        v = fmaf(s6, fmaf(q.e4f, s6, -q.e4f), -q.shiftf);
+        const float s6 = ir2 * ir2 * ir2;
+        v = fmaf(s6, fmaf(q.a12f, s6, -q.a6f), -q.shiftf);
     } else if (KIND == POT_HARMONIC) {
         const float m = d - 1.0f;
         v = d <= 0.0f ? 10e10f : m * m;
@@ -212,8 +215,8 @@ JMM_HD float phi32(const Pot &q, float d, bool in)
 
 // Per-trial constants of the FP32 box-length pass (a scaling of every
 // separation by s), rounded from FP64:
-//   LJ      c12 = 4 eps (s^-12 - 1), c6 = 4 eps (s^-6 - 1), so that
-//           phi(s d) - phi(d) = i6 (c12 i6 - c6) with i6 = (sigma/d)^6
+//   LJ      c12 = 4 eps sigma^12 (s^-12 - 1), c6 = 4 eps sigma^6 (s^-6 - 1),
+//           so that phi(s d) - phi(d) = s6 (c12 s6 - c6) with s6 = d^-6
 //   Yukawa  sm1 = s - 1, is = 1/s, kx = -kappa (s - 1): phi(s d) - phi(d) =
 //           phi(d) (expm1(kx d) - sm1) is, expm1 by its Taylor series while
 //           |kx| (cutoff + step) <= 0.1 (poly = 1), else both separations
@@ -237,9 +240,9 @@ JMM_HD float box_term32(const Pot &q, const BoxK &bk, float f0, float f1, bool i
     float dlt, r0;
     if (KIND == POT_LJ) {
         const float ir2 = frcp(f0 * f0);
-        const float i6 = ir2 * ir2 * ir2 * q.sig6f;
-        dlt = i6 * fmaf(bk.c12, i6, -bk.c6);
-        r0 = fmaf(i6, fmaf(q.e4f, i6, -q.e4f), -q.shiftf);
+        const float s6 = ir2 * ir2 * ir2;
+        dlt = s6 * fmaf(bk.c12, s6, -bk.c6);
+        r0 = fmaf(s6, fmaf(q.a12f, s6, -q.a6f), -q.shiftf);
     } else if (KIND == POT_YUKAWA) {
         const float e0 = q.p0f * fexp(-q.p1f * f0) * frcp(f0);
         if (bk.poly) {

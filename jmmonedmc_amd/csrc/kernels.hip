@@ -87,43 +87,33 @@ __device__ __forceinline__ double delta_e64(const Pot &q, const double *X, int64
 }
 
 // FP32 form: separations formed in FP64 (positions are FP64), cutoff decided
-// in FP64, pair energies in FP32, Kahan-compensated sum.
+// in FP64, pair energies in FP32, each side's terms summed in FP64 (exact
+// sums of the FP32 terms; an FP64 add issues at the FP32 rate on gfx950).
 template <int KIND>
 __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64_t i, int64_t N,
                                             bool per, double L, uint32_t w, double xi, double xn,
                                             uint32_t &terms)
 {
-    float s = 0.0f, c = 0.0f;
+    double left = 0.0, right = 0.0;
     for (uint32_t k = 1; k <= w; ++k) {
         double xj;
         if (!nbpos(X, i - (int64_t)k, N, per, L, xj)) break;
         const double dold = xi - xj, dnew = xn - xj;
         if (dold > q.rc && dnew > q.rc) break;
         const bool io = inside64<KIND>(q, dold), in = inside64<KIND>(q, dnew);
-        const float t = phi32<KIND>(q, (float)dnew, in) - phi32<KIND>(q, (float)dold, io);
-        const float y = t - c;
-        const float tt = s + y;
-        c = (tt - s) - y;
-        s = tt;
+        left += (double)(phi32<KIND>(q, (float)dnew, in) - phi32<KIND>(q, (float)dold, io));
         terms += (io ? 1u : 0u) + (in ? 1u : 0u);
     }
-    const double left = (double)s - (double)c;
-    s = 0.0f;
-    c = 0.0f;
     for (uint32_t k = 1; k <= w; ++k) {
         double xj;
         if (!nbpos(X, i + (int64_t)k, N, per, L, xj)) break;
         const double dold = xj - xi, dnew = xj - xn;
         if (dold > q.rc && dnew > q.rc) break;
         const bool io = inside64<KIND>(q, dold), in = inside64<KIND>(q, dnew);
-        const float t = phi32<KIND>(q, (float)dnew, in) - phi32<KIND>(q, (float)dold, io);
-        const float y = t - c;
-        const float tt = s + y;
-        c = (tt - s) - y;
-        s = tt;
+        right += (double)(phi32<KIND>(q, (float)dnew, in) - phi32<KIND>(q, (float)dold, io));
         terms += (io ? 1u : 0u) + (in ? 1u : 0u);
     }
-    return left + ((double)s - (double)c);
+    return left + right;
 }
 
 // Box-length trial energy over the pairs whose left member is in [p0, p1),
@@ -132,10 +122,10 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
 // restating fav's trial energy (src/jmmMCState.cpp:2175-2235) as a
 // difference per pair, each partner walk ending at the first partner beyond
 // the cutoff at both separations: FP64 form phi64(s d) - phi64(d); FP32 form
-// box_term32 of the two separations (Kahan-compensated), or the exact
-// change in the number of pairs inside the square well.  FP32 form: u0
-// receives the same pairs' energy at the unscaled separations (FP32 pair
-// energies summed in FP64; exact well counts for the square well).
+// box_term32 of the two separations (summed in FP64), or the exact change
+// in the number of pairs inside the square well.  FP32 form: u0 receives
+// the same pairs' energy at the unscaled separations (FP32 pair energies
+// summed in FP64; exact well counts for the square well).
 template <typename Real, int KIND>
 __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, double gmin, const double *X,
                                                int64_t N, bool per, double L, double Lp, double sc,
@@ -143,7 +133,6 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, dou
                                                uint32_t &bad, double &u0)
 {
     double part = 0.0;
-    float s = 0.0f, c = 0.0f;
     int wc = 0, n0 = 0;
     u0 = 0.0;
     for (uint32_t p = p0; p < p1 && !bad; p += stride) {
@@ -177,18 +166,14 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, dou
                     continue;
                 }
                 float e0;
-                const float t = box_term32<KIND>(q, bk, (float)d0, (float)d1, in0, in1, e0);
+                part += (double)box_term32<KIND>(q, bk, (float)d0, (float)d1, in0, in1, e0);
                 u0 += (double)e0;
-                const float y = t - c;
-                const float tt = s + y;
-                c = (tt - s) - y;
-                s = tt;
             }
         }
     }
-    if (sizeof(Real) == 8) return part;
-    if (KIND == POT_HRSW) u0 = -q.p1 * (double)n0;
-    return KIND == POT_HRSW ? -q.p1 * (double)wc : (double)s - (double)c;
+    if (sizeof(Real) == 8 || KIND != POT_HRSW) return part;
+    u0 = -q.p1 * (double)n0;
+    return -q.p1 * (double)wc;
 }
 
 // FP64 full energy of the pairs whose left member is in [p0, p1) (ECheck's
@@ -283,9 +268,9 @@ __device__ __noinline__ VolProp vol_propose(const DevChain &ch, double L, uint64
     }
     v.valid = v.Lp > 0.0 && (!(ch.flags & F_PERIODIC) || v.Lp > ch.Lmin);
     // s^-12 - 1, s^-6 - 1 and s - 1 from expm1 of ln s: no cancellation
-    const double e4 = ch.pot.e4;
-    v.bk.c12 = (float)(e4 * expm1(-12.0 * v.lns));
-    v.bk.c6 = (float)(e4 * expm1(-6.0 * v.lns));
+    const double a6 = ch.pot.e4 * ch.pot.sig6;
+    v.bk.c12 = (float)(a6 * ch.pot.sig6 * expm1(-12.0 * v.lns));
+    v.bk.c6 = (float)(a6 * expm1(-6.0 * v.lns));
     const double sm1 = expm1(v.lns);
     v.bk.sm1 = (float)sm1;
     v.bk.is = (float)(1.0 / v.sc);
@@ -626,8 +611,10 @@ __device__ __forceinline__ bool walk_in(const Pot &, double, double, bool io, bo
 // cutoff from both positions.  FP64 form: the oracle's operations exactly.
 // FP32 form: separations formed in FP64, cutoff decided in FP64 (the FP64
 // set-up and recompute energies count the same pairs), pair energies in FP32
-// without branches, Kahan-compensated side sums; the loop-exit test runs on
-// the separations in FP64 as in the FP64 form.  Partner k's address is read
+// without branches, each side's FP32 terms summed in FP64 (on gfx950 an FP64
+// add issues at the FP32 rate: exact sums of the FP32 terms for fewer
+// instructions than a compensated FP32 sum with its freeze selects); the
+// loop-exit test runs on the separations in FP64 as in the FP64 form.  Partner k's address is read
 // on every lane of the wave, live or not: for k <= w it lies in the chain's
 // image (ghosts or halos are wg >= w wide, blocks longer than w), and a lane
 // that is not live drops the value.  Returns the decision; dE receives the
@@ -636,15 +623,15 @@ __device__ __forceinline__ bool walk_in(const Pot &, double, double, bool io, bo
 // PAIR = false: one lane walks both sides.  PAIR = true: two adjacent lanes
 // share the trial, side 0 (even lane) walking the left partners and side 1
 // the right; each forms its side's sum exactly as the one-lane walk does
-// (same terms, same order; a side's FP32 sum stops changing when its walk
-// ends), then the two sums are exchanged and added.  Addition commutes, so
+// (same terms, same order; once a side's walk ends its terms are zero and
+// adding them is exact), then the two sums are exchanged and added.  Addition commutes, so
 // both lanes hold the one-lane result bit for bit.  Both lanes of a pair
 // must be active together.
 //
 // W2 (FP32, one lane): two partners per side per loop iteration, four
 // independent pair evaluations in flight instead of two and half the
-// loop-exit tests; the terms are added in the same order with the same
-// freeze rule, so the result is the one-step walk's bit for bit.
+// loop-exit tests; the terms are added in the same order, so the result is
+// the one-step walk's bit for bit.
 template <typename Real, int KIND, bool PAIR = false, bool W2 = false>
 __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, double gmin,
                                           double beta, float betaf, uint32_t i, uint32_t t,
@@ -696,7 +683,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             const double bf = exp(-dE * beta);
             return go && (dE <= 0.0 || bf > u);
         }
-        float sS = 0.0f, cS = 0.0f;
+        double sS = 0.0;  // the side's FP32 pair terms summed in FP64
         int wS = 0;  // hard rod + square well: the well count, as the one-lane walk
         uint32_t k = 1;
         if (__any(live)) {
@@ -707,21 +694,18 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
             const double o = left ? xi - xj : xj - xi, nn = left ? xn - xj : xj - xn;
             const bool io = inside64<KIND>(q, o), inn = inside64<KIND>(q, nn);
             const bool in = live && walk_in<KIND>(q, o, nn, io, inn);
-            const float tt = phi32<KIND>(q, (float)nn, inn) - phi32<KIND>(q, (float)o, io);
             if (KIND == POT_HRSW) {
                 wS += in ? (int)inn - (int)io : 0;
             } else {
-                const float y = tt - cS;
-                const float uu = sS + y;
-                cS = in ? (uu - sS) - y : cS;
-                sS = in ? uu : sS;
+                // terms of partners off the walk are zero: exact no-op adds
+                sS += (double)(phi32<KIND>(q, (float)nn, in && inn) - phi32<KIND>(q, (float)o, in && io));
             }
             terms += n_walked<KIND>(in, io, inn);
             live = in && k < kS;
             ++k;
         } while (__any(live));
         }
-        const double mine = KIND == POT_HRSW ? -q.p1 * (double)wS : (double)sS - (double)cS;
+        const double mine = KIND == POT_HRSW ? -q.p1 * (double)wS : sS;
         dE = mine + __shfl_xor(mine, 1);
         const float bf = fexp(-betaf * (float)dE);
         return go && (dE <= 0.0 || (double)bf > u);
@@ -754,11 +738,12 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const double bf = exp(-dE * beta);
         return go && (dE <= 0.0 || bf > u);
     }
-    float sL = 0.0f, cL = 0.0f, sR = 0.0f, cR = 0.0f;
+    // each side's FP32 pair terms summed in FP64 (exact; the terms of
+    // partners off a side's walk are zero, exact no-op adds)
+    double sL = 0.0, sR = 0.0;
     // hard rod + square well: every pair term is -eps or 0, so each side's
     // change is -eps times the change in the number of partners inside the
-    // well, counted exactly (for eps = 1 the compensated FP32 sums give the
-    // same value)
+    // well, counted exactly
     int wL = 0, wR = 0;
     uint32_t k = 1;
     if (W2) {
@@ -779,8 +764,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const bool ioR = inside64<KIND>(q, oR), inR64 = inside64<KIND>(q, nR);
         const bool inL = liveL && walk_in<KIND>(q, oL, nL, ioL, inL64);
         const bool inR = liveR && walk_in<KIND>(q, oR, nR, ioR, inR64);
-        const float tL = phi32<KIND>(q, (float)nL, inL64) - phi32<KIND>(q, (float)oL, ioL);
-        const float tR = phi32<KIND>(q, (float)nR, inR64) - phi32<KIND>(q, (float)oR, ioR);
+        const float tL = phi32<KIND>(q, (float)nL, inL && inL64) - phi32<KIND>(q, (float)oL, inL && ioL);
+        const float tR = phi32<KIND>(q, (float)nR, inR && inR64) - phi32<KIND>(q, (float)oR, inR && ioR);
         const bool liveL2 = inL && k < kL, liveR2 = inR && k < kR;
         const double oL2 = xi - xjL2, nL2 = xn - xjL2;
         const double oR2 = xjR2 - xi, nR2 = xjR2 - xn;
@@ -788,32 +773,16 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const bool ioR2 = inside64<KIND>(q, oR2), inR642 = inside64<KIND>(q, nR2);
         const bool inL2 = liveL2 && walk_in<KIND>(q, oL2, nL2, ioL2, inL642);
         const bool inR2 = liveR2 && walk_in<KIND>(q, oR2, nR2, ioR2, inR642);
-        const float tL2 = phi32<KIND>(q, (float)nL2, inL642) - phi32<KIND>(q, (float)oL2, ioL2);
-        const float tR2 = phi32<KIND>(q, (float)nR2, inR642) - phi32<KIND>(q, (float)oR2, ioR2);
+        const float tL2 = phi32<KIND>(q, (float)nL2, inL2 && inL642) - phi32<KIND>(q, (float)oL2, inL2 && ioL2);
+        const float tR2 = phi32<KIND>(q, (float)nR2, inR2 && inR642) - phi32<KIND>(q, (float)oR2, inR2 && ioR2);
         if (KIND == POT_HRSW) {
             wL += (inL ? (int)inL64 - (int)ioL : 0) + (inL2 ? (int)inL642 - (int)ioL2 : 0);
             wR += (inR ? (int)inR64 - (int)ioR : 0) + (inR2 ? (int)inR642 - (int)ioR2 : 0);
         } else {
-        {
-            const float yL = tL - cL;
-            const float uL = sL + yL;
-            cL = inL ? (uL - sL) - yL : cL;
-            sL = inL ? uL : sL;
-            const float yR = tR - cR;
-            const float uR = sR + yR;
-            cR = inR ? (uR - sR) - yR : cR;
-            sR = inR ? uR : sR;
-        }
-        {
-            const float yL = tL2 - cL;
-            const float uL = sL + yL;
-            cL = inL2 ? (uL - sL) - yL : cL;
-            sL = inL2 ? uL : sL;
-            const float yR = tR2 - cR;
-            const float uR = sR + yR;
-            cR = inR2 ? (uR - sR) - yR : cR;
-            sR = inR2 ? uR : sR;
-        }
+            sL += (double)tL;
+            sR += (double)tR;
+            sL += (double)tL2;
+            sR += (double)tR2;
         }
         terms += n_walked<KIND>(inL, ioL, inL64) + n_walked<KIND>(inR, ioR, inR64) +
                  n_walked<KIND>(inL2, ioL2, inL642) + n_walked<KIND>(inR2, ioR2, inR642);
@@ -838,21 +807,14 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
         const bool ioR = inside64<KIND>(q, oR), inR64 = inside64<KIND>(q, nR);
         const bool inL = liveL && walk_in<KIND>(q, oL, nL, ioL, inL64);
         const bool inR = liveR && walk_in<KIND>(q, oR, nR, ioR, inR64);
-        const float tL = phi32<KIND>(q, (float)nL, inL64) - phi32<KIND>(q, (float)oL, ioL);
-        const float tR = phi32<KIND>(q, (float)nR, inR64) - phi32<KIND>(q, (float)oR, ioR);
+        const float tL = phi32<KIND>(q, (float)nL, inL && inL64) - phi32<KIND>(q, (float)oL, inL && ioL);
+        const float tR = phi32<KIND>(q, (float)nR, inR && inR64) - phi32<KIND>(q, (float)oR, inR && ioR);
         if (KIND == POT_HRSW) {
             wL += inL ? (int)inL64 - (int)ioL : 0;
             wR += inR ? (int)inR64 - (int)ioR : 0;
         } else {
-            // compensated sums; a side whose walk has ended keeps its sum
-            const float yL = tL - cL;
-            const float uL = sL + yL;
-            cL = inL ? (uL - sL) - yL : cL;
-            sL = inL ? uL : sL;
-            const float yR = tR - cR;
-            const float uR = sR + yR;
-            cR = inR ? (uR - sR) - yR : cR;
-            sR = inR ? uR : sR;
+            sL += (double)tL;
+            sR += (double)tR;
         }
         terms += n_walked<KIND>(inL, ioL, inL64) + n_walked<KIND>(inR, ioR, inR64);
         liveL = inL && k < kL;
@@ -861,8 +823,7 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
     } while (__any(liveL || liveR));
     }
     }
-    dE = KIND == POT_HRSW ? -q.p1 * (double)wL + -q.p1 * (double)wR
-                          : ((double)sL - (double)cL) + ((double)sR - (double)cR);
+    dE = KIND == POT_HRSW ? -q.p1 * (double)wL + -q.p1 * (double)wR : sL + sR;
     const float bf = fexp(-betaf * (float)dE);
     return go && (dE <= 0.0 || (double)bf > u);
 }
@@ -871,8 +832,8 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 // over the pairs (i, i+k) whose left member is in the lane's block, in the
 // order t = 0.., k = 1.. : the energy change of scaling the configuration by
 // vp.sc pair by pair, FP64 form phi64(s d) - phi64(d), FP32 form box_term32
-// (one reciprocal per LJ pair through the scaling identity) summed with
-// Kahan compensation; each particle's walk ends at its first partner beyond
+// (one reciprocal per LJ pair through the scaling identity) summed in FP64;
+// each particle's walk ends at its first partner beyond
 // the cutoff at both the unscaled and the scaled separation (the FP64
 // inside-the-cutoff flags decide in both forms, so both sum the same pairs).
 // The walk over (particle, partner) is one
@@ -898,11 +859,10 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
                                            Wrap wrap, uint32_t &bad, double &u0)
 {
     const double *X = c.R;
-    double part = 0.0;
-    float sf = 0.0f, cc = 0.0f;
+    double part = 0.0;  // FP64 terms, or the FP32 form's FP32 terms summed in FP64
     int wc = 0;  // hard rod + square well, FP32: pairs inside the well (energy -eps each)
     int n0 = 0;  // ... and at the unscaled separations
-    u0 = 0.0;
+    u0 = 0.0;    // FP32 form: the pairs' FP32 energy at the unscaled separations
     bool run = on && size > 0;
     uint32_t t = 0, k = 1;
     double xi = X[run ? a0 : 0u];
@@ -944,19 +904,14 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
             wc += (add ? (int)in1 - (int)in0 : 0) + (add2 ? (int)jn1 - (int)jn0 : 0);
             n0 += (add && in0 ? 1 : 0) + (add2 && jn0 ? 1 : 0);
         } else {
+            // flags masked by add / add2: pairs off the walk give zeros
             float g0, g1;
-            const float tt = box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, in0, in1, g0);
-            const float tt2 = box_term32<KIND>(q, vp.bk, (float)e0, (float)e1, jn0, jn1, g1);
-            u0 += add ? (double)g0 : 0.0;
-            u0 += add2 ? (double)g1 : 0.0;
-            const float y = tt - cc;
-            const float z = sf + y;
-            cc = add ? (z - sf) - y : cc;
-            sf = add ? z : sf;
-            const float y2 = tt2 - cc;
-            const float z2 = sf + y2;
-            cc = add2 ? (z2 - sf) - y2 : cc;
-            sf = add2 ? z2 : sf;
+            const float tt = box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, add && in0, add && in1, g0);
+            const float tt2 = box_term32<KIND>(q, vp.bk, (float)e0, (float)e1, add2 && jn0, add2 && jn1, g1);
+            part += (double)tt;
+            part += (double)tt2;
+            u0 += (double)g0;
+            u0 += (double)g1;
         }
         bad |= isbad ? 1u : 0u;
         x1 = k == 1 ? xj : x1;
@@ -996,12 +951,8 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
             n0 += add && in0 ? 1 : 0;
         } else {
             float g0;
-            const float tt = box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, in0, in1, g0);
-            u0 += add ? (double)g0 : 0.0;
-            const float y = tt - cc;
-            const float z = sf + y;
-            cc = add ? (z - sf) - y : cc;
-            sf = add ? z : sf;
+            part += (double)box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, add && in0, add && in1, g0);
+            u0 += (double)g0;
         }
         bad |= isbad ? 1u : 0u;
         x1 = k == 1 ? xj : x1;
@@ -1018,9 +969,9 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     } while (__any(run));
     }
     }
-    if (sizeof(Real) == 8) return part;
-    if (KIND == POT_HRSW) u0 = -q.p1 * (double)n0;
-    return KIND == POT_HRSW ? -q.p1 * (double)wc : (double)sf - (double)cc;
+    if (sizeof(Real) == 8 || KIND != POT_HRSW) return part;
+    u0 = -q.p1 * (double)n0;
+    return -q.p1 * (double)wc;
 }
 
 // Virial and hypervirial (FP64) of the pairs whose left member is in the

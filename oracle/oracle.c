@@ -420,7 +420,7 @@ int orc_vol_accept(double dU, double dL, double ln_s, double P, double beta,
  *
  * The FP32 form evaluates pair energies in single precision from separations
  * formed and tested against the cutoff in double precision, and sums them
- * with compensation.  Its decision can differ from the double-precision one
+ * in double precision.  Its decision can differ from the double-precision one
  * only where the margin of the decision lies within the error of its energy
  * change.  Bound of one single-precision pair evaluation at a separation d
  * inside the cutoff, u = 2^-24:

@@ -507,14 +507,14 @@ def test_hist_hand_binned_periodic(oracle):
 
 def _phi32_lj(d, eps=1.0, sig=1.0, shift=0.0):
     """The kernel's FP32 LJ sequence (phi32) in numpy float32: d rounded to
-    single, 1/d^2, (1/d^2)^3 sigma^6, two fused steps (rounded separately
-    here, which only widens the error)."""
+    single, 1/d^2, s6 = (1/d^2)^3, s6 (4 eps sigma^12 s6 - 4 eps sigma^6) -
+    shift in two fused steps (rounded separately here, which only widens the
+    error)."""
     f = np.float32
     d = f(d)
     ir2 = f(1.0) / (d * d)
-    s6 = ir2 * ir2 * ir2 * f(sig ** 6)
-    e4 = f(4.0 * eps)
-    return s6 * (e4 * s6 - e4) - f(shift)
+    s6 = ir2 * ir2 * ir2
+    return s6 * (f(4.0 * eps * sig ** 12) * s6 - f(4.0 * eps * sig ** 6)) - f(shift)
 
 
 def _phi32_yukawa(d, A=1.0, k=0.2):
