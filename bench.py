@@ -54,6 +54,10 @@ def parse():
                         "(-1 = the workload's sweep count: 1e4, C5 2e3)")
     p.add_argument("--acc-sweeps", type=int, default=-1,
                    help="sweeps of each accuracy run (0 = no accuracy check, -1 = as --acc-burnin)")
+    p.add_argument("--acc-replicas", type=int, default=16,
+                   help="FP64 reference runs of the accuracy check")
+    p.add_argument("--acc-dump", default="",
+                   help="write every accuracy run's per-chain values to this .npz file")
     return p.parse_args()
 
 
@@ -135,29 +139,33 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
     return out
 
 
-GATE = ("U", "L", "rho")
-
-
-def accuracy_check(chains, opts, burnin, sweeps, bs):
+def accuracy_check(chains, opts, burnin, sweeps, bs, replicas, dump=""):
     """FP32 against FP64 per chain and observable at stated accuracy.
 
     A burn-in of `burnin` FP64 sweeps from the lattice start gives every
-    chain a common starting configuration; three runs of `sweeps` sweeps
-    start from it with independent seeds: an FP64 control, a second FP64
-    run (the null) and the FP32 form.  Per chain and run: <U>, <L>, <rho>
-    with standard errors sqrt(2 tau_int var / n), the fluctuation
-    properties cP and betaT (block formulas), and tau_int of U.  z-scores of
-    the null and of the FP32 run against the control, over the chains whose
-    estimates rest on at least 30 independent samples (the rest counted as
-    excluded).  The gate is valid when the null has at least 99 % of its
-    chains within 3 standard errors for <U>, <L> and <rho>; it passes when
-    the FP32 run does too.  The FP32 error per observable is stated as its
-    median and RMS relative difference from the control beside the null's
-    (the resolution of the comparison)."""
+    chain a common starting configuration.  From it, `replicas` + 2 runs of
+    `sweeps` sweeps with independent seeds: `replicas` FP64 reference runs,
+    one more FP64 run (the null) and one FP32 run.  Per run and chain, the
+    values of <U>, <L>, <rho>, cP, betaT and tau_int of U (stats.run_values;
+    tau_int compared through its logarithm).
+    The null and the FP32 run are each compared with the reference runs
+    (stats.replica_compare): t = (x - mean) / (sd sqrt(1 + 1/K)) per chain,
+    Student's t with K - 1 degrees of freedom if the run is one more draw
+    of the reference runs' distribution, judged against the t band of the
+    probability of a normal 3 sigma band (99.73 %).  The standard errors
+    come from the runs' spread, so slow modes that an autocorrelation
+    estimate from one run misses are counted.  The gate is valid when the
+    null has at least 99 % of its chains within the band for every
+    observable, and passes when the FP32 run does too.  The FP32 error per
+    observable: mean t (bias, with its standard error), the median
+    difference in units of one run's spread, and relative differences, each
+    beside the null's."""
     import dataclasses
     from jmmonedmc_amd import stats
+    from jmmonedmc_amd import _native as nat
     from jmmonedmc_amd.engine import Engine
     t0 = time.perf_counter()
+    workers = max(1, min(16, len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16"))))
     print(f"[bench] accuracy: {burnin} FP64 burn-in sweeps", file=sys.stderr, flush=True)
     with Engine(chains, form="batched_fp64", sample_capacity=bs, block_sweeps=bs, **opts) as e:
         for _ in range(max(1, burnin // bs)):
@@ -167,32 +175,44 @@ def accuracy_check(chains, opts, burnin, sweeps, bs):
     N = np.array([c.N for c in chains])
     P = np.array([c.P for c in chains])
     T = np.array([c.T for c in chains])
-    obs = {}
-    for name, form, off in (("fp64_control", "batched_fp64", 1_000_003), ("fp64_null", "batched_fp64", 2_000_003),
-                            ("fp32", "batched_fp32", 3_000_003)):
-        print(f"[bench] accuracy: {name} run of {sweeps} sweeps", file=sys.stderr, flush=True)
-        chs = [dataclasses.replace(c, seed=c.seed + off) for c in chains]
+    pot = np.array([c.potential for c in chains])
+    names = {nat.POT_LJ: "lj", nat.POT_HRSW: "hrsw", nat.POT_YUKAWA: "yukawa", nat.POT_HARMONIC: "harmonic"}
+    groups = {names[p]: pot == p for p in np.unique(pot)}
+    runs = [(f"ref{j}", "batched_fp64") for j in range(replicas)] + [("null", "batched_fp64"),
+                                                                     ("fp32", "batched_fp32")]
+    vals = {}
+    pending = None
+    for j, (name, form) in enumerate(runs):
+        print(f"[bench] accuracy: run {name} ({form}, {sweeps} sweeps)", file=sys.stderr, flush=True)
+        chs = [dataclasses.replace(c, seed=c.seed + (j + 1) * 1_000_003) for c in chains]
         with Engine(chs, form=form, sample_capacity=sweeps, block_sweeps=bs, **opts) as e:
             e.set_states(x, L0)
             e.run(sweeps)
+            if pending is not None:  # the previous run's values while this one runs
+                vals[pending[0]] = stats.run_values(pending[1], pending[2], N, P, T, workers=workers)
             smp = e.read_samples(sweeps)
-        obs[name] = stats.chain_observables_batch(smp["U"], smp["L"], N, P, T)
+        pending = (name, np.ascontiguousarray(smp["U"]), np.ascontiguousarray(smp["L"]))
         del smp
-    null = stats.compare_runs(obs["fp64_control"], obs["fp64_null"])
-    fp32 = stats.compare_runs(obs["fp64_control"], obs["fp32"])
-    valid = all(null[k]["frac_within_3sigma"] is not None and null[k]["frac_within_3sigma"] >= 0.99
-                for k in GATE)
-    passed = valid and all(fp32[k]["frac_within_3sigma"] is not None and fp32[k]["frac_within_3sigma"] >= 0.99
-                           for k in GATE)
-    per_obs = {}
-    for k in stats.OBSERVABLES:
-        per_obs[k] = {"fp32_vs_fp64": fp32[k], "null_fp64_vs_fp64": null[k]}
-    return {"method": "common start after an FP64 burn-in, three runs with independent seeds (FP64 control, "
-                      "FP64 null, FP32); per chain, mean with SE sqrt(2 tau_int var/n); z against the control "
-                      "over chains with >= 30 independent samples",
-            "burnin_sweeps": burnin, "sweeps_per_run": sweeps, "chains": len(chains),
-            "gate_observables": list(GATE), "gate_valid_null_ge_0.99": bool(valid),
-            "gate_passed": bool(passed), "observables": per_obs,
+    vals[pending[0]] = stats.run_values(pending[1], pending[2], N, P, T, workers=workers)
+    del pending
+    if dump:
+        np.savez_compressed(dump, N=N, P=P, T=T, potential=pot,
+                            **{f"{r}/{k}": v for r, d in vals.items() for k, v in d.items()})
+    ref = [vals[f"ref{j}"] for j in range(replicas)]
+    null = stats.replica_compare(ref, vals["null"], groups=groups)
+    fp32 = stats.replica_compare(ref, vals["fp32"], groups=groups)
+    valid = all(null[k]["frac_within_band"] is not None and null[k]["frac_within_band"] >= 0.99
+                for k in stats.OBSERVABLES)
+    passed = valid and all(fp32[k]["frac_within_band"] is not None and fp32[k]["frac_within_band"] >= 0.99
+                           for k in stats.OBSERVABLES)
+    per_obs = {k: {"fp32_vs_fp64": fp32[k], "null_fp64_vs_fp64": null[k]} for k in stats.OBSERVABLES}
+    return {"method": f"common start after an FP64 burn-in; {replicas} FP64 reference runs, one FP64 null run "
+                      f"and one FP32 run with independent seeds; per chain t = (x - mean_ref) / "
+                      f"(sd_ref sqrt(1 + 1/{replicas})), Student's t with {replicas - 1} degrees of freedom "
+                      f"under the hypothesis, band = its two-sided 99.73 % quantile (the probability of 3 sigma)",
+            "burnin_sweeps": burnin, "sweeps_per_run": sweeps, "reference_runs": replicas,
+            "chains": len(chains), "band": null["U"]["band"], "gate_observables": list(stats.OBSERVABLES),
+            "gate_valid_null_ge_0.99": bool(valid), "gate_passed": bool(passed), "observables": per_obs,
             "seconds": time.perf_counter() - t0}
 
 
@@ -294,7 +314,8 @@ def main():
     acc_sw = a.acc_sweeps if a.acc_sweeps >= 0 else acc_burn
     if acc_sw > 0:
         accuracy = leg("accuracy runs (GPU, untimed)",
-                       lambda: accuracy_check(chains, opts, acc_burn, acc_sw, a.block_sweeps))
+                       lambda: accuracy_check(chains, opts, acc_burn, acc_sw, a.block_sweeps,
+                                                      a.acc_replicas, a.acc_dump))
 
     # roofline of the dominant kernel (the batched FP32 sweep kernel)
     achieved = fp32["flops"] / (fp32["kernel_ms"] * 1e-3) / 1e12
@@ -361,7 +382,7 @@ def main():
         "value_status": (None if accuracy is None else
                          "accuracy gate passed" if accuracy["gate_passed"] else
                          "accuracy gate failed" if accuracy["gate_valid_null_ge_0.99"] else
-                         "accuracy gate not valid (the FP64 null is below 99 % within 3 sigma)"),
+                         "accuracy gate not valid (the FP64 null is below 99 % within the 3 sigma band)"),
         "roofline": roof,
         "cpu_baseline": cpu,
         "accuracy_fp32_vs_fp64": accuracy,

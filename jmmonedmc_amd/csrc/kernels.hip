@@ -122,10 +122,12 @@ __device__ __forceinline__ double delta_e32(const Pot &q, const double *X, int64
 // restating fav's trial energy (src/jmmMCState.cpp:2175-2235) as a
 // difference per pair, each partner walk ending at the first partner beyond
 // the cutoff at both separations: FP64 form phi64(s d) - phi64(d); FP32 form
-// box_term32 of the two separations (summed in FP64), or the exact change
-// in the number of pairs inside the square well.  FP32 form: u0 receives
-// the same pairs' energy at the unscaled separations (FP32 pair energies
-// summed in FP64; exact well counts for the square well).
+// box_term32 of the two separations (summed in FP64).  FP32 form: u0
+// receives the same pairs' energy at the unscaled separations (FP32 pair
+// energies summed in FP64); for the square well the return value is the
+// scaled configuration's energy instead (-eps times its pairs inside the
+// well, each walk ending at the first scaled partner outside), from which
+// the caller subtracts the running energy (exact, as in vol_lane).
 template <typename Real, int KIND>
 __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, double gmin, const double *X,
                                                int64_t N, bool per, double L, double Lp, double sc,
@@ -133,7 +135,7 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, dou
                                                uint32_t &bad, double &u0)
 {
     double part = 0.0;
-    int wc = 0, n0 = 0;
+    int wc = 0;
     u0 = 0.0;
     for (uint32_t p = p0; p < p1 && !bad; p += stride) {
         const double xi = X[p], xis = xi * sc;
@@ -161,8 +163,8 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, dou
                 const bool in0 = inside64<KIND>(q, d0), in1 = inside64<KIND>(q, d1);
                 if (!(in0 || in1)) break;
                 if (KIND == POT_HRSW) {
-                    wc += (int)in1 - (int)in0;
-                    n0 += (int)in0;
+                    if (!in1) break;
+                    ++wc;
                     continue;
                 }
                 float e0;
@@ -172,7 +174,6 @@ __device__ __forceinline__ double scaled_delta(const Pot &q, const BoxK &bk, dou
         }
     }
     if (sizeof(Real) == 8 || KIND != POT_HRSW) return part;
-    u0 = -q.p1 * (double)n0;
     return -q.p1 * (double)wc;
 }
 
@@ -476,8 +477,11 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
             bad = k2[1];
             // FP32 form: the box-length pass summed the configuration's FP32
             // energy over every pair; it replaces the running sum, so FP32
-            // trial errors do not accumulate between recomputes
-            if (sizeof(Real) == 4 && npt && vp.valid && !bad) U = v[2];
+            // trial errors do not accumulate between recomputes.  Square
+            // well: the pass gave the scaled energy, the running energy is
+            // exact
+            if (sizeof(Real) == 4 && KIND == POT_HRSW) vpart = vpart - U;
+            else if (sizeof(Real) == 4 && npt && vp.valid && !bad) U = v[2];
         }
         acc_total += acc;
         uint32_t accv = 0;
@@ -846,9 +850,13 @@ __device__ __forceinline__ bool trial_lds(const LdsChain &c, const Pot &q, doubl
 // scaled position are kept rather than read again.  `last`: the lane holds
 // the chain's last block of a periodic chain, whose partners past the end
 // are the first particles shifted by L; wrap(m) returns particle m's
-// unshifted position (scaled and shifted by Lp here).  The hard-rod +
-// square-well FP32 energy change is -eps times the change in the number of
-// pairs inside the well, counted exactly.  W2: two partners of
+// unshifted position (scaled and shifted by Lp here).  Hard rod + square
+// well, FP32 form: the lane returns the scaled configuration's energy, -eps
+// times the number of its scaled pairs inside the well (each walk ends at
+// the first scaled partner outside), and the caller subtracts the running
+// energy, which is -eps times the unscaled count exactly (every trial and
+// recompute changes it by whole pairs), so the energy change is the one the
+// pairwise sum gives.  W2: two partners of
 // a particle per iteration (for the common walk of one partner inside and
 // the next beyond the cutoff, one iteration per particle), same terms in
 // the same order.
@@ -858,10 +866,10 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
                                            uint32_t size, uint32_t a0, uint32_t gapR, bool last,
                                            Wrap wrap, uint32_t &bad, double &u0)
 {
+    constexpr bool WELL = sizeof(Real) == 4 && KIND == POT_HRSW;
     const double *X = c.R;
     double part = 0.0;  // FP64 terms, or the FP32 form's FP32 terms summed in FP64
-    int wc = 0;  // hard rod + square well, FP32: pairs inside the well (energy -eps each)
-    int n0 = 0;  // ... and at the unscaled separations
+    int wc = 0;  // hard rod + square well, FP32: scaled pairs inside the well
     u0 = 0.0;    // FP32 form: the pairs' FP32 energy at the unscaled separations
     bool run = on && size > 0;
     uint32_t t = 0, k = 1;
@@ -890,19 +898,18 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         const bool in0 = inside64<KIND>(q, d0), in1 = inside64<KIND>(q, d1);
         const bool jn0 = inside64<KIND>(q, e0), jn1 = inside64<KIND>(q, e1);
         const bool isbad = valid && k == 1 && d1 < gmin;
-        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : !(in0 || in1));
+        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : WELL ? !in1 : !(in0 || in1));
         const bool add = valid && !isbad && !stop;
         const bool valid2 = add && k < kR;
-        const bool stop2 = valid2 && (sizeof(Real) == 8 ? e0 > q.rc && e1 > q.rc : !(jn0 || jn1));
+        const bool stop2 = valid2 && (sizeof(Real) == 8 ? e0 > q.rc && e1 > q.rc : WELL ? !jn1 : !(jn0 || jn1));
         const bool add2 = valid2 && !stop2;
         if (sizeof(Real) == 8) {
             const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             const double tt2 = phi64<KIND>(q, e1) - phi64<KIND>(q, e0);
             part = add ? part + tt : part;
             part = add2 ? part + tt2 : part;
-        } else if (KIND == POT_HRSW) {
-            wc += (add ? (int)in1 - (int)in0 : 0) + (add2 ? (int)jn1 - (int)jn0 : 0);
-            n0 += (add && in0 ? 1 : 0) + (add2 && jn0 ? 1 : 0);
+        } else if (WELL) {
+            wc += (add && in1 ? 1 : 0) + (add2 && jn1 ? 1 : 0);
         } else {
             // flags masked by add / add2: pairs off the walk give zeros
             float g0, g1;
@@ -941,14 +948,13 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
         const double d0 = xj - xi, d1 = xjs - xis;
         const bool in0 = inside64<KIND>(q, d0), in1 = inside64<KIND>(q, d1);
         const bool isbad = valid && k == 1 && d1 < gmin;
-        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : !(in0 || in1));
+        const bool stop = valid && (sizeof(Real) == 8 ? d0 > q.rc && d1 > q.rc : WELL ? !in1 : !(in0 || in1));
         const bool add = valid && !isbad && !stop;
         if (sizeof(Real) == 8) {
             const double tt = phi64<KIND>(q, d1) - phi64<KIND>(q, d0);
             part = add ? part + tt : part;
-        } else if (KIND == POT_HRSW) {
-            wc += add ? (int)in1 - (int)in0 : 0;
-            n0 += add && in0 ? 1 : 0;
+        } else if (WELL) {
+            wc += add && in1 ? 1 : 0;
         } else {
             float g0;
             part += (double)box_term32<KIND>(q, vp.bk, (float)d0, (float)d1, add && in0, add && in1, g0);
@@ -969,9 +975,7 @@ __device__ __forceinline__ double vol_lane(const LdsChain &c, const Pot &q, doub
     } while (__any(run));
     }
     }
-    if (sizeof(Real) == 8 || KIND != POT_HRSW) return part;
-    u0 = -q.p1 * (double)n0;
-    return -q.p1 * (double)wc;
+    return WELL ? -q.p1 * (double)wc : part;
 }
 
 // Virial and hypervirial (FP64) of the pairs whose left member is in the
@@ -1128,8 +1132,11 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
             bad = k2[1];
             // FP32 form: the box-length pass summed the configuration's FP32
             // energy over every pair; it replaces the running sum, so FP32
-            // trial errors do not accumulate between recomputes
-            if (sizeof(Real) == 4 && npt && vp.valid && !bad) U = v[2];
+            // trial errors do not accumulate between recomputes.  Square
+            // well: the pass gave the scaled energy, the running energy is
+            // exact
+            if (sizeof(Real) == 4 && KIND == POT_HRSW) vpart = vpart - U;
+            else if (sizeof(Real) == 4 && npt && vp.valid && !bad) U = v[2];
         }
         acc_total += acc;
         uint32_t accv = 0;
@@ -1330,7 +1337,7 @@ __device__ __forceinline__ uint64_t seg_pidx(uint32_t j, uint32_t par, uint32_t 
 // the final counters (all).  Partial sums are double-buffered by sweep
 // parity: a segment is at most one all-segment point ahead of another.
 template <typename Real, int KIND, int TW, int LPB>
-__device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
+__device__ __forceinline__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
                             BatchedScratch<TW> &sc, uint32_t soff)
 {
     const SweepArgs &a = g.a;
@@ -1571,14 +1578,15 @@ __device__ void run_segment(const SegArgs &g, const DevSeg &sg, double *smem,
         gather_int(par, 3, 0);
         U = U + tot[0];
         // FP32 form: the configuration's FP32 energy from the box-length
-        // pass replaces the running sum (as in the whole-chain kernel)
-        if (sizeof(Real) == 4 && npt && vp.valid && tot[3] == 0.0) U = tot[8];
+        // pass replaces the running sum (as in the whole-chain kernel);
+        // square well: the pass gave the scaled energy
+        if (sizeof(Real) == 4 && KIND != POT_HRSW && npt && vp.valid && tot[3] == 0.0) U = tot[8];
         acc = (uint32_t)tot[2];
         acc_total += acc;
         uint32_t accv = 0;
         if (npt) {
             ++n_vt;
-            const double dUv = tot[1];
+            const double dUv = (sizeof(Real) == 4 && KIND == POT_HRSW) ? tot[1] - U : tot[1];
             if (vp.valid && tot[3] == 0.0 && vol_accept(dUv, vp.dL, vp.lns, P, beta, vp.jac, vp.u)) {
                 if (own)
                     for (uint32_t t = 0; t < size; ++t) c.R[a0 + t] = c.R[a0 + t] * vp.sc;

@@ -309,6 +309,127 @@ def fluctuations_from_samples(U, L, P: float, T: float, nblocks: int = 20):
     return out
 
 
+def _tau_rows_fast(X, c: float = 6.0, chunk: int = 512, workers: int = 8):
+    """_tau_rows's tau for many rows, FFTs on `workers` threads."""
+    import scipy.fft as sfft
+    X = np.asarray(X, dtype=np.float64)
+    nc, n = X.shape
+    out = np.empty(nc)
+    m = sfft.next_fast_len(2 * n, real=True)
+    W = np.arange(1, n)
+    for lo in range(0, nc, chunk):
+        Y = X[lo:lo + chunk]
+        Y = Y - Y.mean(axis=1, keepdims=True)
+        F = sfft.rfft(Y, m, axis=1, workers=workers)
+        A = sfft.irfft(F * np.conj(F), m, axis=1, workers=workers)[:, :n]
+        c0 = A[:, :1].copy()
+        c0[c0 == 0] = 1.0
+        cum = 0.5 + np.cumsum((A / c0)[:, 1:], axis=1)
+        ok = W[None, :] >= c * cum
+        k = np.where(ok.any(axis=1), ok.argmax(axis=1), n - 2)
+        out[lo:lo + chunk] = np.maximum(cum[np.arange(Y.shape[0]), k], 0.5)
+    return out
+
+
+def run_values(U, L, N, P, T, nblocks: int = 20, workers: int = 8):
+    """One run's per-chain values of every observable in OBSERVABLES, from
+    per-sweep samples U, L [chain, sweep]: <U>, <L>, <rho> = <N/L>; cP and
+    betaT as chain_observables defines them (the formulas of
+    scripts/Analyze_Mean.py:164-176 per block of n/nblocks sweeps, averaged
+    over the blocks); tau_int of U (Sokal's window, c = 6).  Returns
+    {observable: array[chain]}."""
+    U = np.asarray(U, dtype=np.float64)
+    L = np.asarray(L, dtype=np.float64)
+    nc, n = U.shape
+    N = np.broadcast_to(np.asarray(N, dtype=np.float64), (nc,))
+    P = np.broadcast_to(np.asarray(P, dtype=np.float64), (nc,))
+    T = np.broadcast_to(np.asarray(T, dtype=np.float64), (nc,))
+    out = {"U": U.mean(axis=1), "L": L.mean(axis=1), "rho": (N[:, None] / L).mean(axis=1)}
+    b = n // nblocks
+    if b >= 2:
+        def bm(x):
+            return x[:, : b * nblocks].reshape(nc, nblocks, b).mean(axis=2)
+        f = fluctuation_properties(bm(U), bm(U * U), bm(L), bm(L * L), bm(L * U), P[:, None], T[:, None])
+        out["cP"] = f["cP"].mean(axis=1)
+        out["betaT"] = f["betaT"].mean(axis=1)
+    else:
+        out["cP"] = np.full(nc, np.nan)
+        out["betaT"] = np.full(nc, np.nan)
+    out["tau_U"] = _tau_rows_fast(U, workers=workers)
+    return out
+
+
+def replica_band(K: int, q: float = 0.9973) -> float:
+    """Half-width of the two-sided probability-q band of Student's t with
+    K - 1 degrees of freedom (q = 0.9973: the probability of a normal
+    variable's 3 sigma band)."""
+    from scipy import stats as st
+    return float(st.t.ppf(0.5 + q / 2.0, K - 1))
+
+
+# compared on a log scale: an estimate of tau_int from one run is skewed
+# (a run whose window takes in a slow fluctuation reads several times the
+# typical value), its logarithm is close to normal
+LOG_COMPARED = ("tau_U",)
+
+
+def replica_compare(ref, test, q: float = 0.9973, groups=None):
+    """One run's per-chain values against K independent reference runs of
+    the same chains from the same starting configuration.
+
+    ref: K >= 3 run_values() results; test: one.  Per chain and observable
+    t = (x - m) / (s sqrt(1 + 1/K)), m and s the mean and standard deviation
+    over the reference runs.  If the test run is one more draw from the
+    distribution the reference runs sample, and a run's value is normal, t
+    follows Student's t with K - 1 degrees of freedom: the standard error
+    comes from the runs themselves, not from a model of the
+    autocorrelation.  Observables in LOG_COMPARED are compared through
+    their logarithms (then the relative differences below are of the
+    logarithms).  Returns {observable: summary}: the chains compared
+    (finite values, s > 0), the fraction with |t| within the probability-q
+    band (`band`), mean t with its standard error (a bias of the test run
+    shows as a mean t away from 0), the standard deviation of t (expected
+    sqrt((K-1)/(K-3)) under the hypothesis), the largest |t|, the median
+    |x - m| / s (the difference in units of one run's spread) and the median
+    and RMS of |x - m| / |m|.  groups: optional {name: bool mask[chain]},
+    the fraction and mean t per group as well."""
+    K = len(ref)
+    if K < 3:
+        raise ValueError("need at least 3 reference runs")
+    band = replica_band(K, q)
+    res = {}
+    for k in OBSERVABLES:
+        R = np.stack([np.asarray(r[k], dtype=np.float64) for r in ref])
+        x = np.asarray(test[k], dtype=np.float64)
+        if k in LOG_COMPARED:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                R, x = np.log(R), np.log(x)
+        m = R.mean(axis=0)
+        sd = R.std(axis=0, ddof=1)
+        ok = np.isfinite(x) & np.isfinite(m) & np.isfinite(sd) & (sd > 0)
+        t = (x[ok] - m[ok]) / (sd[ok] * math.sqrt(1.0 + 1.0 / K))
+        nz = ok & (m != 0)
+        rd = np.abs(x[nz] - m[nz]) / np.abs(m[nz]) if nz.any() else np.zeros(1)
+        r = dict(chains_compared=int(t.size), chains_excluded=int((~ok).sum()), band=band,
+                 frac_within_band=float((np.abs(t) <= band).mean()) if t.size else None,
+                 mean_t=float(t.mean()) if t.size else None,
+                 mean_t_se=float(t.std() / math.sqrt(t.size)) if t.size > 1 else None,
+                 std_t=float(t.std()) if t.size else None,
+                 max_abs_t=float(np.abs(t).max()) if t.size else None,
+                 median_diff_over_sd=float(np.median(np.abs(x[ok] - m[ok]) / sd[ok])) if t.size else None,
+                 median_rel_diff=float(np.median(rd)), rms_rel_diff=float(np.sqrt((rd ** 2).mean())))
+        if groups:
+            tt = np.full(len(x), np.nan)
+            tt[ok] = t
+            for g, mask in groups.items():
+                sel = np.asarray(mask, bool) & ok
+                if sel.any():
+                    r[g] = dict(chains=int(sel.sum()), frac_within_band=float((np.abs(tt[sel]) <= band).mean()),
+                                mean_t=float(tt[sel].mean()))
+        res[k] = r
+    return res
+
+
 def mean_sem(x, nblocks: int = 20):
     """Mean and block SEM with a fixed number of blocks."""
     x = np.asarray(x, dtype=np.float64)

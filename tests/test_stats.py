@@ -228,3 +228,74 @@ def test_chain_observables_batch_matches_single():
         for k in stats.OBSERVABLES:
             for j in range(3):
                 assert bt[k][j][c] == pytest.approx(one[k][j], rel=1e-9, abs=1e-12), (c, k, j)
+
+
+def test_replica_band_closed_forms():
+    """The t band against closed forms of Student's t quantile: one degree
+    of freedom (Cauchy), tan(pi (p - 1/2)); two, (2p - 1) sqrt(2 / (4p(1-p)));
+    and the normal 3 sigma limit for many degrees of freedom."""
+    p = 0.5 + 0.9973 / 2
+    assert stats.replica_band(2) == pytest.approx(math.tan(math.pi * (p - 0.5)), rel=1e-9)
+    assert stats.replica_band(3) == pytest.approx((2 * p - 1) * math.sqrt(2.0 / (4 * p * (1 - p))), rel=1e-9)
+    assert stats.replica_band(100000) == pytest.approx(3.0, abs=2e-3)
+
+
+def test_replica_compare_null_and_planted_bias():
+    """replica_compare on synthetic per-chain values: a test run drawn like
+    the K reference runs lies within the band at the band's probability and
+    its t has the standard deviation sqrt((K-1)/(K-3)) of Student's t; a
+    test run shifted by two of its own spreads fails, with mean t that of
+    the noncentral t of 2 / sqrt(1 + 1/K); chains whose reference runs do not vary are excluded
+    and counted; the per-group summaries split the chains."""
+    rng = np.random.default_rng(24)
+    K, nc = 16, 20000
+    mu = rng.uniform(-100, 100, nc)
+    sd = rng.uniform(0.1, 10, nc)
+
+    def draw(shift=0.0):
+        z = rng.standard_normal(nc)
+        v = mu + shift * sd + sd * z
+        out = {k: v for k in stats.OBSERVABLES}
+        # log-compared observables: normal on the log scale
+        for k in stats.LOG_COMPARED:
+            out[k] = np.exp(mu / 50.0 + (shift + z) * sd / 20.0)
+        return out
+
+    ref = [draw() for _ in range(K)]
+    null = stats.replica_compare(ref, draw(), groups={"first": np.arange(nc) < 100})
+    for k in stats.OBSERVABLES:
+        r = null[k]
+        assert r["chains_compared"] == nc and r["chains_excluded"] == 0
+        assert r["band"] == pytest.approx(stats.replica_band(K))
+        assert abs(r["frac_within_band"] - 0.9973) < 5 * math.sqrt(0.0027 * 0.9973 / nc)
+        assert r["std_t"] == pytest.approx(math.sqrt((K - 1) / (K - 3)), rel=0.03)
+        assert abs(r["mean_t"]) < 4 * r["mean_t_se"]
+        assert r["first"]["chains"] == 100
+    bad = stats.replica_compare(ref, draw(shift=2.0))
+    # noncentral t: mean delta sqrt(nu/2) Gamma((nu-1)/2) / Gamma(nu/2)
+    nu, delta = K - 1, 2.0 / math.sqrt(1 + 1 / K)
+    want = delta * math.sqrt(nu / 2) * math.exp(math.lgamma((nu - 1) / 2) - math.lgamma(nu / 2))
+    assert bad["U"]["mean_t"] == pytest.approx(want, abs=0.03)
+    assert bad["U"]["frac_within_band"] < 0.95
+    flat = [dict(r) for r in ref]
+    for r in flat:
+        r["L"] = np.ones(nc)
+    ex = stats.replica_compare(flat, draw())
+    assert ex["L"]["chains_compared"] == 0 and ex["L"]["chains_excluded"] == nc
+    with pytest.raises(ValueError):
+        stats.replica_compare(ref[:2], draw())
+
+
+def test_run_values_match_chain_observables():
+    """The per-run values of many chains equal chain_observables' values
+    chain by chain (means, block fluctuation formulas, tau_int of U)."""
+    rng = np.random.default_rng(25)
+    nc, n = 5, 3000
+    U = np.cumsum(rng.standard_normal((nc, n)), axis=1) * 0.01 + rng.standard_normal((nc, n)) - 40
+    L = 200 + rng.standard_normal((nc, n)) + np.sin(np.arange(n) / 50.0)
+    N, P, T = np.arange(50, 55), np.linspace(0.2, 1.0, nc), np.linspace(0.6, 1.8, nc)
+    v = stats.run_values(U, L, N, P, T, workers=2)
+    for c in range(nc):
+        one = stats.chain_observables(U[c], L[c], int(N[c]), P[c], T[c])
+        for k in stats.OBSERVABLES:
+            assert v[k][c] == pytest.approx(one[k][0], rel=1e-9, abs=1e-12), (c, k)
