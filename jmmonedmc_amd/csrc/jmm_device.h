@@ -214,26 +214,25 @@ JMM_HD float phi32(const Pot &q, float d, bool in)
 }
 
 // Per-trial constants of the FP32 box-length pass (a scaling of every
-// separation by s), rounded from FP64:
-//   LJ      c12 = 4 eps sigma^12 (s^-12 - 1), c6 = 4 eps sigma^6 (s^-6 - 1),
-//           so that phi(s d) - phi(d) = s6 (c12 s6 - c6) with s6 = d^-6
-//   Yukawa  sm1 = s - 1, is = 1/s, kx = -kappa (s - 1): phi(s d) - phi(d) =
-//           phi(d) (expm1(kx d) - sm1) is, expm1 by its Taylor series while
-//           |kx| (cutoff + step) <= 0.1 (poly = 1), else both separations
-//           are evaluated directly
+// separation by s), rounded from FP64: for LJ c12 = 4 eps sigma^12
+// (s^-12 - 1), c6 = 4 eps sigma^6 (s^-6 - 1), so that phi(s d) - phi(d) =
+// s6 (c12 s6 - c6) with s6 = d^-6 (one reciprocal per pair, no
+// cancellation).
 struct BoxK {
-    float c12, c6, sm1, is, kx;
-    int poly;
+    float c12, c6;
 };
 
 // Change of one pair's FP32 energy when its separation goes from d0 to d1
 // (d1 = s d0 up to the FP64 roundings of the scaled positions), given the
 // FP64 inside-the-cutoff flags of both: a pair entering the cutoff adds
 // phi(d1) - shift, a pair leaving it removes phi(d0) - shift, a pair inside
-// at both changes by the scaling identity's difference.  e0 receives the
-// pair's energy at d0 (phi(d0) - shift, zero outside the cutoff), so the
-// same pass sums the configuration's energy.  The hard rod + square well is
-// handled by the callers (exact counts).
+// at both changes by phi(d1) - phi(d0): for LJ the scaling identity's
+// difference, for Yukawa and the harmonic well the difference of the two
+// FP32 energies (two exponentials per Yukawa pair cost less than a series
+// for the bracket, measured).  e0 receives the pair's energy at d0
+// (phi(d0) - shift, zero outside the cutoff), so the same pass sums the
+// configuration's energy.  The hard rod + square well is handled by the
+// callers (exact counts).
 template <int KIND>
 JMM_HD float box_term32(const Pot &q, const BoxK &bk, float f0, float f1, bool in0, bool in1, float &e0)
 {
@@ -244,20 +243,10 @@ JMM_HD float box_term32(const Pot &q, const BoxK &bk, float f0, float f1, bool i
         dlt = s6 * fmaf(bk.c12, s6, -bk.c6);
         r0 = fmaf(s6, fmaf(q.a12f, s6, -q.a6f), -q.shiftf);
     } else if (KIND == POT_YUKAWA) {
-        const float e0 = q.p0f * fexp(-q.p1f * f0) * frcp(f0);
-        if (bk.poly) {
-            const float x = bk.kx * f0;
-            float p = fmaf(x, 1.0f / 720.0f, 1.0f / 120.0f);
-            p = fmaf(x, p, 1.0f / 24.0f);
-            p = fmaf(x, p, 1.0f / 6.0f);
-            p = fmaf(x, p, 0.5f);
-            p = fmaf(x, p, 1.0f);
-            const float em = x * p;
-            dlt = e0 * ((em - bk.sm1) * bk.is);
-        } else {
-            dlt = q.p0f * fexp(-q.p1f * f1) * frcp(f1) - e0;
-        }
-        r0 = e0 - q.shiftf;
+        const float v0 = q.p0f * fexp(-q.p1f * f0) * frcp(f0) - q.shiftf;
+        const float v1 = q.p0f * fexp(-q.p1f * f1) * frcp(f1) - q.shiftf;
+        e0 = in0 ? v0 : 0.0f;
+        return (in1 ? v1 : 0.0f) - e0;
     } else {
         const float m0 = f0 - 1.0f, m1 = f1 - 1.0f;
         dlt = m1 * m1 - m0 * m0;

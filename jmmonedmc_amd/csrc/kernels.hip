@@ -268,16 +268,10 @@ __device__ __noinline__ VolProp vol_propose(const DevChain &ch, double L, uint64
         v.dL = v.Lp - L;
     }
     v.valid = v.Lp > 0.0 && (!(ch.flags & F_PERIODIC) || v.Lp > ch.Lmin);
-    // s^-12 - 1, s^-6 - 1 and s - 1 from expm1 of ln s: no cancellation
+    // s^-12 - 1 and s^-6 - 1 from expm1 of ln s: no cancellation
     const double a6 = ch.pot.e4 * ch.pot.sig6;
     v.bk.c12 = (float)(a6 * ch.pot.sig6 * expm1(-12.0 * v.lns));
     v.bk.c6 = (float)(a6 * expm1(-6.0 * v.lns));
-    const double sm1 = expm1(v.lns);
-    v.bk.sm1 = (float)sm1;
-    v.bk.is = (float)(1.0 / v.sc);
-    const double kx = -ch.pot.p1 * sm1;
-    v.bk.kx = (float)kx;
-    v.bk.poly = fabs(kx) * (ch.pot.rc + ch.max_step) <= 0.1 ? 1 : 0;
     return v;
 }
 
