@@ -33,8 +33,11 @@ extern "C" {
 
 /* potentials (reference POT keyword, src/readInput.cpp:104-131, plus the
  * configurations' hard-rod/square-well and Yukawa potentials) */
-#define JMM_POT_LJ 0       /* 4 eps((s/d)^12-(s/d)^6); p = {eps, sigma}; inclusive cutoff;
-                              cutoff = INFINITY is the reference's "LJ" */
+#define JMM_POT_LJ 0       /* 4 eps((s/d)^12-(s/d)^6); p = {eps, sigma, exclusion}; inclusive
+                              cutoff; cutoff = INFINITY is the reference's "LJ"; a trial
+                              that would bring two particles closer than the exclusion
+                              distance (0: 0.5 sigma, where e^(-phi/T) < 1e-3500 at T <= 2)
+                              is rejected without evaluating it */
 #define JMM_POT_HARMONIC 1 /* (d-1)^2, 1e11 for d <= 0, strict cutoff (reference HARMONIC) */
 #define JMM_POT_HRSW 2     /* hard rod p0, well depth p1 out to p2 (strict) */
 #define JMM_POT_YUKAWA 3   /* p0 exp(-p1 d)/d, hard core p2, inclusive cutoff */
@@ -84,7 +87,8 @@ typedef struct jmm_engine_opts {
     int32_t device;           /* HIP device ordinal */
     uint32_t sample_capacity; /* per-chain sweeps held on device between reads (0 = 1024) */
     uint32_t block_sweeps;    /* sweeps per kernel launch (0 = 100) */
-    uint32_t wg_threads;      /* batched forms: lanes per workgroup, 64..1024 (0 = 256) */
+    uint32_t wg_threads;      /* batched forms: lanes per workgroup, rounded up to 256, 512 or
+                                 1024 and to the largest chain's block count (0 = 256) */
     uint32_t use_graph;       /* 1: replay sweep blocks from a captured hipGraph */
     uint32_t log_accepts;     /* 1: record one byte per displacement trial (tests) */
     uint32_t lds_budget;      /* batched forms: LDS bytes per workgroup (0 = 64 KiB) */

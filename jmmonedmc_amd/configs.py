@@ -30,9 +30,17 @@ def c1(boundary=nat.PERIODIC, seed=1):
                   max_step=0.5, **LJ)]
 
 
-def c2(n_chains=256, N=4096):
+def c2(n_chains=256, N=4096, exclusion=0.0, bcap=0):
+    """256 NPT LJ chains of 4096.  exclusion > 0: the LJ exclusion distance
+    (a trial closer than it is rejected; the default is 0.5 sigma) and bcap
+    the largest block count; exclusion 0.75 with bcap 1024 gives 512 blocks
+    per chain instead of 256 (M_min = floor((3 + 2 * 0.5) / 0.75) + 1 = 6);
+    trials landing closer than 0.75 sigma, whose pair energy is at least
+    phi(0.75) = 103.8 eps (Boltzmann factor 8e-46 at T = 1), are then
+    rejected without evaluation."""
+    lj = dict(LJ, pot_p=(LJ["pot_p"][0], LJ["pot_p"][1], exclusion, 0.0))
     return [Chain(N=N, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT, vol_move=nat.VOL_LOG, max_dv=0.01,
-                  max_step=0.5, seed=s, chain_id=s, **LJ) for s in range(n_chains)]
+                  max_step=0.5, seed=s, chain_id=s, bcap=bcap, **lj) for s in range(n_chains)]
 
 
 def c3_params(n_chains=4096, seed=C3_PARAM_SEED):

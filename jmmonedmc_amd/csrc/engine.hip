@@ -61,7 +61,7 @@ int fail(int code, const char *fmt, ...)
 double gmin_of(const jmm_chain_spec &s)
 {
     switch (s.potential) {
-    case JMM_POT_LJ: return 0.5 * s.pot_p[1];
+    case JMM_POT_LJ: return s.pot_p[2] > 0.0 ? s.pot_p[2] : 0.5 * s.pot_p[1];
     case JMM_POT_HRSW: return s.pot_p[0];
     case JMM_POT_YUKAWA: return s.pot_p[2];
     default: return std::numeric_limits<double>::denorm_min();  // harmonic: gap <= 0
@@ -123,6 +123,8 @@ int validate(const jmm_chain_spec &s, int idx)
         if (s.cutoff != s.pot_p[2]) return fail(JMM_EINVAL, "chain %d: hard-rod/square-well cutoff must equal the well edge", idx);
     }
     if (s.potential == JMM_POT_YUKAWA && !(s.pot_p[2] > 0.0)) return fail(JMM_EINVAL, "chain %d: Yukawa hard core must be > 0", idx);
+    if (s.potential == JMM_POT_LJ && (s.pot_p[2] < 0.0 || s.pot_p[2] >= s.cutoff))
+        return fail(JMM_EINVAL, "chain %d: LJ exclusion distance must be 0 (0.5 sigma) or inside the cutoff", idx);
     if (s.boundary == JMM_PERIODIC) {
         if (!std::isfinite(s.cutoff)) return fail(JMM_EINVAL, "chain %d: periodic chains need a finite cutoff", idx);
         if (!(s.L0 > 2.0 * (s.cutoff + s.max_step)))
@@ -644,7 +646,10 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
     if (e->form == JMM_FORM_PLAIN_FP64) {
         std::iota(order.begin(), order.end(), 0u);
     } else {
-        e->T = (int)std::max<uint32_t>(o.wg_threads ? o.wg_threads : 256u, maxB) <= 256 ? 256 : 1024;
+        // workgroup width: 256, 512 or 1024 lanes, at least the largest
+        // chain's block count (1024 lanes cap the kernel at 128 VGPRs)
+        const uint32_t want = std::max<uint32_t>(o.wg_threads ? o.wg_threads : 256u, maxB);
+        e->T = want <= 256 ? 256 : want <= 512 ? 512 : 1024;
         if (maxB > 1024) return bail(fail(JMM_EINVAL, "jmm_create: more than 1024 blocks in a chain"));
         size_t budget = o.lds_budget ? o.lds_budget : 65536;
         budget = std::min<size_t>(budget, (size_t)lds_max);
@@ -779,7 +784,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             CREATE_CHK(hipMemcpy(Gb.d_bundles, bv.data(), sizeof(DevBundle) * Gb.n_items, hipMemcpyHostToDevice));
         }
     }
-    // segmented chains: one cooperative launch, one workgroup per segment
+    // segmented chains: one launch, one workgroup per segment, all resident
     {
         std::vector<DevSeg> segv;
         uint32_t wgmax = 1, bsmax = 1;
@@ -831,7 +836,7 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
             const uint32_t zero = 0;
             CREATE_CHK(hipMemcpy(e->d_seg_err, &zero, sizeof zero, hipMemcpyHostToDevice));
             e->seg_ctrl = (uint32_t)e->grp.size();
-            e->use_graph = 0;  // cooperative launches are not captured
+            e->use_graph = 0;  // split chains are launched directly, not from a graph
         }
     }
     e->n_ctrl = (uint32_t)e->grp.size() + (e->n_segs ? 1u : 0u);

@@ -96,7 +96,7 @@ struct ObsArgs {
 };
 
 // A chain too large for one workgroup's LDS is split into S segments of Bs
-// consecutive blocks, one workgroup each (cooperative launch: all resident).
+// consecutive blocks, one workgroup each (one launch, all resident).
 struct DevSeg {
     uint32_t chain;  // internal chain index
     uint32_t s, S;   // segment index, segments of the chain

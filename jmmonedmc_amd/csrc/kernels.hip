@@ -2158,6 +2158,8 @@ static const void *pick_shape(int T, int gx)
 {
     if (T == 1024) return gx ? (const void *)k_sweep_batched<Real, 1024, true>
                              : (const void *)k_sweep_batched<Real, 1024, false>;
+    if (T == 512) return gx ? (const void *)k_sweep_batched<Real, 512, true>
+                            : (const void *)k_sweep_batched<Real, 512, false>;
     return gx ? (const void *)k_sweep_batched<Real, 256, true>
               : (const void *)k_sweep_batched<Real, 256, false>;
 }
@@ -2210,11 +2212,18 @@ const void *segmented_kernel(int fp32, int tw, int lpb)
     return fp32 ? (const void *)k_sweep_seg<float, 256, 1> : (const void *)k_sweep_seg<double, 256, 1>;
 }
 
+// An ordinary launch: the segments' synchronisation needs every workgroup
+// resident at once, which jmm_create checks against the occupancy of the
+// whole device (the launch follows the stream's previous work, so the
+// device is free when it starts), and a wait that never completes ends
+// after its bounded polls with the error flag set.  A cooperative launch
+// would add only a runtime check of the same residency, and a rocprofv3
+// kernel trace of a program using one ended in a segmentation fault with
+// no results written.
 hipError_t launch_segmented(const SegArgs &a, int fp32, int tw, int lpb, size_t lds, hipStream_t s)
 {
     void *args[] = {(void *)&a};
-    return hipLaunchCooperativeKernel(segmented_kernel(fp32, tw, lpb), dim3(a.n_segs), dim3(tw), args,
-                                      (unsigned int)lds, s);
+    return hipLaunchKernel(segmented_kernel(fp32, tw, lpb), dim3(a.n_segs), dim3(tw), args, lds, s);
 }
 
 hipError_t launch_hist(const HistArgs &a, hipStream_t s)

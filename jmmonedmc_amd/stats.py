@@ -367,6 +367,15 @@ def replica_band(K: int, q: float = 0.9973) -> float:
     return float(st.t.ppf(0.5 + q / 2.0, K - 1))
 
 
+def band_allowance(n: int, q: float = 0.9973, level: float = 0.99) -> int:
+    """How many of n chains may lie outside a probability-q band before a
+    comparison fails: 1 % of them, or, where that is less, the `level`
+    quantile of the binomial count outside (n, 1 - q) (with 64 chains one
+    outside is likely under the hypothesis, 1 % of them is none)."""
+    from scipy import stats as st
+    return int(max(math.floor(0.01 * n), st.binom.ppf(level, n, 1.0 - q)))
+
+
 # compared on a log scale: an estimate of tau_int from one run is skewed
 # (a run whose window takes in a slow fluctuation reads several times the
 # typical value), its logarithm is close to normal
@@ -387,7 +396,8 @@ def replica_compare(ref, test, q: float = 0.9973, groups=None):
     their logarithms (then the relative differences below are of the
     logarithms).  Returns {observable: summary}: the chains compared
     (finite values, s > 0), the fraction with |t| within the probability-q
-    band (`band`), mean t with its standard error (a bias of the test run
+    band (`band`), the number outside it and how many band_allowance
+    permits, mean t with its standard error (a bias of the test run
     shows as a mean t away from 0), the standard deviation of t (expected
     sqrt((K-1)/(K-3)) under the hypothesis), the largest |t|, the median
     |x - m| / s (the difference in units of one run's spread) and the median
@@ -412,6 +422,7 @@ def replica_compare(ref, test, q: float = 0.9973, groups=None):
         rd = np.abs(x[nz] - m[nz]) / np.abs(m[nz]) if nz.any() else np.zeros(1)
         r = dict(chains_compared=int(t.size), chains_excluded=int((~ok).sum()), band=band,
                  frac_within_band=float((np.abs(t) <= band).mean()) if t.size else None,
+                 outside=int((np.abs(t) > band).sum()), allowed_outside=band_allowance(t.size, q),
                  mean_t=float(t.mean()) if t.size else None,
                  mean_t_se=float(t.std() / math.sqrt(t.size)) if t.size > 1 else None,
                  std_t=float(t.std()) if t.size else None,

@@ -7,9 +7,13 @@ the samples are bit-identical).  For a given chain mix the tuner times a
 short run of every candidate with the engine's own HIP-event kernel timer
 and keeps the fastest.
 
-Candidates: the LDS budget per workgroup (how many chains share one), the
-workgroup width (256 or 1024 lanes; 1024 admits chains of up to 1024
-blocks), and whether sweep blocks replay a captured hipGraph.  Results can be
+Candidates (candidates_for): the LDS budget per workgroup (how many chains
+share one), the workgroup width (256, 512 or 1024 lanes: wider workgroups
+admit chains of up to that many blocks and put more blocks in one; 1024
+lanes cap the kernel at 128 VGPRs, so they spill),
+whether sweep blocks replay a captured hipGraph, and, for chains too large
+for one workgroup's LDS, the split threshold (how many workgroups share a
+chain) and the lanes per block of a split chain (1 or 2).  Results can be
 cached in a JSON file keyed by a signature of the chain mix and the device.
 """
 from __future__ import annotations
@@ -22,6 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from ._native import JmmError
 from .engine import Chain, Engine
 
 DEFAULT_CANDIDATES = [
@@ -33,10 +38,31 @@ DEFAULT_CANDIDATES = [
 ]
 
 
+# a chain whose LDS image (8 bytes per particle and its ghosts) exceeds
+# this many bytes may be split over several workgroups
+SPLIT_CANDIDATE_BYTES = 96 * 1024
+
+
+def candidates_for(chains) -> list:
+    """DEFAULT_CANDIDATES, the 512- and 1024-lane workgroups at two LDS
+    budgets each, and,
+    when a chain may be split, split thresholds of 64 and 32 KiB per
+    workgroup and two lanes per block of a split chain."""
+    cands = [dict(c) for c in DEFAULT_CANDIDATES]
+    cands += [dict(lds_budget=65536, wg_threads=512, use_graph=True),
+              dict(lds_budget=98304, wg_threads=512, use_graph=True),
+              dict(lds_budget=65536, wg_threads=1024, use_graph=True),
+              dict(lds_budget=98304, wg_threads=1024, use_graph=True)]
+    if any(8 * c.N > SPLIT_CANDIDATE_BYTES for c in chains):
+        base = dict(lds_budget=65536, wg_threads=256, use_graph=True)
+        cands += [dict(base, seg_lanes=2), dict(base, seg_lds=65536), dict(base, seg_lds=32768)]
+    return cands
+
+
 @dataclass
 class TuneResult:
     best: dict
-    ms_per_sweep: dict = field(default_factory=dict)   # candidate key -> ms per sweep
+    ms_per_sweep: dict = field(default_factory=dict)   # candidate key -> ms per sweep (None: refused)
     signature: str = ""
 
 
@@ -67,13 +93,19 @@ def tune(chains, form: str = "batched_fp32", candidates=None, block_sweeps: int 
         if sig in stored:
             r = stored[sig]
             return TuneResult(best=r["best"], ms_per_sweep=r["ms_per_sweep"], signature=sig)
-    cands = candidates or DEFAULT_CANDIDATES
+    cands = candidates or candidates_for(chains)
     times = {}
     for c in cands:
         opts = dict(c)
         cap = (blocks + 1) * block_sweeps
-        with Engine(chains, form=form, device=device, sample_capacity=cap,
-                    block_sweeps=block_sweeps, **opts) as e:
+        try:
+            e = Engine(chains, form=form, device=device, sample_capacity=cap, block_sweeps=block_sweeps, **opts)
+        except JmmError:
+            # a candidate the device cannot hold (split chains whose
+            # workgroups would not all be resident) is refused at creation
+            times[_key(c)] = None
+            continue
+        with e:
             e.run(block_sweeps)
             e.sync()
             e.kernel_time(reset=True)
@@ -81,7 +113,10 @@ def tune(chains, form: str = "batched_fp32", candidates=None, block_sweeps: int 
             e.sync()
             n, ms, sw = e.kernel_time(reset=True)
         times[_key(c)] = ms / max(sw, 1)
-    best = min(cands, key=lambda c: times[_key(c)])
+    ok = [c for c in cands if times[_key(c)] is not None]
+    if not ok:
+        raise JmmError("tune: no candidate could be created for these chains")
+    best = min(ok, key=lambda c: times[_key(c)])
     res = TuneResult(best=dict(best), ms_per_sweep=times, signature=sig)
     if cache:
         stored = {}

@@ -267,7 +267,7 @@ static inline uint64_t n_inside(const orc_spec *s, double dold, double dnew)
 static double gmin_of(const orc_spec *s)
 {
     switch (s->potential) {
-    case ORC_POT_LJ: return 0.5 * s->pot_p[1];
+    case ORC_POT_LJ: return s->pot_p[2] > 0.0 ? s->pot_p[2] : 0.5 * s->pot_p[1];
     case ORC_POT_HRSW: return s->pot_p[0];
     case ORC_POT_YUKAWA: return s->pot_p[2];
     default: return DBL_TRUE_MIN; /* harmonic: reject gap <= 0 */

@@ -46,6 +46,12 @@ CASES = {
     "npt_walls_linear": lambda: [Chain(N=100, L0=200.0, T=1.0, P=0.5, ensemble=nat.NPT,
                                        boundary=nat.WALLS, vol_move=nat.VOL_LINEAR, max_dv=1.0,
                                        seed=11, **lj(shifted=0))],
+    # NPT periodic LJ with the exclusion distance at 0.75 sigma and up to
+    # 1024 blocks: 512 blocks of 6, so a 512-lane workgroup (the C2 variant
+    # with twice the blocks, smaller)
+    "lj_exclusion_npt": lambda: [Chain(N=3100, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT,
+                                       vol_move=nat.VOL_LOG, max_dv=0.01, seed=8, chain_id=4, bcap=1024,
+                                       **lj(pot_p=(1.0, 1.0, 0.75, 0.0)))],
     # hard rod + square well, NPT periodic (C3 second half)
     "hrsw_npt": lambda: [Chain(N=200, rho=0.5, T=1.0, P=1.0, ensemble=nat.NPT, max_dv=0.01,
                                seed=5, **hrsw())],

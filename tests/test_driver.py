@@ -122,6 +122,27 @@ def test_schedule_and_validation_without_gpu():
     assert b"periodic length" in lib.jmm_last_error() or b"device" in lib.jmm_last_error()
 
 
+def test_lj_exclusion_distance_schedule(oracle):
+    """The LJ exclusion distance sets the schedule in the engine and the
+    oracle alike: C2's chain at 0.5 sigma (default) has M_min = 9 and 256
+    blocks; at 0.75 sigma M_min = floor(4 / 0.75) + 1 = 6, w = floor(3.5 /
+    0.75) = 4, and with bcap 1024 the 4096 particles take 512 blocks.  An
+    exclusion outside the cutoff is refused."""
+    import ctypes as C
+    from jmmonedmc_amd import _native
+    from jmmonedmc_amd.configs import c2
+    from jmmonedmc_amd.engine import Chain, schedule
+    from helpers import oracle_spec
+    base, wide = c2(1)[0], c2(1, exclusion=0.75, bcap=1024)[0]
+    assert tuple(schedule(base)) == (256, 9, 7) == tuple(oracle.schedule(oracle_spec(oracle, base)))
+    assert tuple(schedule(wide)) == (512, 6, 4) == tuple(oracle.schedule(oracle_spec(oracle, wide)))
+    lib = _native.lib()
+    s = Chain(N=10, L0=40.0, cutoff=3.0, pot_p=(1.0, 1.0, 3.5, 0.0)).to_c()
+    h = C.c_void_p()
+    assert lib.jmm_create(C.byref(s), 1, None, C.byref(h)) != 0
+    assert b"exclusion" in lib.jmm_last_error()
+
+
 @pytest.mark.gpu
 def test_driver_replays_oracle(gpu, oracle, driver, tmp_path):
     """The driver on a C1-shaped INPUT (periodic, shifted LJ, NVT): the final

@@ -235,11 +235,13 @@ def test_autotuner_candidates_give_identical_chains(gpu, tmp_path):
     """The tuned parameters only change packing and launch: every candidate
     produces bit-identical samples and states (FP32 form, mixed batch), and
     the tuner returns one of them, cached by the chain-mix signature."""
-    from jmmonedmc_amd.tune import DEFAULT_CANDIDATES, tune
+    from jmmonedmc_amd.tune import candidates_for, tune
     print("[gpu] autotuner invariance", flush=True)
     chains = CASES["mixed_batch"]() + [Chain(N=2000, L0=4000.0, seed=9, chain_id=77, **lj())]
+    cands = candidates_for(chains)
+    assert any(c["wg_threads"] == 1024 for c in cands)
     ref = None
-    for cand in DEFAULT_CANDIDATES:
+    for cand in cands:
         with Engine(chains, form="batched_fp32", sample_capacity=40, block_sweeps=20, **cand) as e:
             e.run(40)
             smp = e.read_samples(40)
@@ -252,6 +254,6 @@ def test_autotuner_candidates_give_identical_chains(gpu, tmp_path):
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), cand
     cache = str(tmp_path / "tune.json")
     r1 = tune(chains, block_sweeps=20, blocks=1, cache=cache)
-    assert r1.best in DEFAULT_CANDIDATES and len(r1.ms_per_sweep) == len(DEFAULT_CANDIDATES)
+    assert r1.best in cands and len(r1.ms_per_sweep) == len(cands)
     r2 = tune(chains, block_sweeps=20, blocks=1, cache=cache)
     assert r2.best == r1.best and r2.ms_per_sweep == r1.ms_per_sweep

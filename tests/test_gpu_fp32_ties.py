@@ -192,6 +192,6 @@ def test_c3_all_chains_fp32_properties(gpu):
 
 @pytest.mark.gpu
 def test_c5_all_chains_fp32_properties(gpu):
-    """All 64 C5 chains (split chains, cooperative launch) for 150 sweeps
+    """All 64 C5 chains (split chains, four workgroups each) for 150 sweeps
     (recompute at 100)."""
     _full_size_properties(configs.c5(), 150, "c5 all chains")

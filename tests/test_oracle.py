@@ -190,7 +190,7 @@ def test_schedule_invariants(oracle, name):
         N = ch.N
         if B > 1:
             assert N // B >= M                      # every block at least M_min long
-        g = {nat.POT_LJ: 0.5 * ch.pot_p[1], nat.POT_HRSW: ch.pot_p[0],
+        g = {nat.POT_LJ: ch.pot_p[2] if ch.pot_p[2] > 0 else 0.5 * ch.pot_p[1], nat.POT_HRSW: ch.pot_p[0],
              nat.POT_YUKAWA: ch.pot_p[2]}.get(ch.potential, 0.0)
         if ch.nbn >= 0:
             assert w <= ch.nbn and M <= ch.nbn + 1

@@ -299,3 +299,14 @@ def test_run_values_match_chain_observables():
         one = stats.chain_observables(U[c], L[c], int(N[c]), P[c], T[c])
         for k in stats.OBSERVABLES:
             assert v[k][c] == pytest.approx(one[k][0], rel=1e-9, abs=1e-12), (c, k)
+
+
+def test_band_allowance():
+    """1 % of the chains for many chains; for few, the 99 % quantile of the
+    binomial count outside a 99.73 % band (64 chains: P(more than 2
+    outside) < 1 %, P(more than 1) > 1 %)."""
+    assert stats.band_allowance(4096) == 40
+    q = 0.0027
+    p_le = lambda k, n: sum(math.comb(n, j) * q ** j * (1 - q) ** (n - j) for j in range(k + 1))
+    assert p_le(1, 64) < 0.99 <= p_le(2, 64)
+    assert stats.band_allowance(64) == 2
