@@ -395,7 +395,9 @@ def replica_compare(ref, test, q: float = 0.9973, groups=None):
     autocorrelation.  Observables in LOG_COMPARED are compared through
     their logarithms (then the relative differences below are of the
     logarithms).  Returns {observable: summary}: the chains compared
-    (finite values, s > 0), the fraction with |t| within the probability-q
+    (finite values, s > 0; chains whose reference runs agree to 1e-12
+    relative are counted as constant and not compared), the fraction with
+    |t| within the probability-q
     band (`band`), the number outside it and how many band_allowance
     permits, mean t with its standard error (a bias of the test run
     shows as a mean t away from 0), the standard deviation of t (expected
@@ -416,11 +418,16 @@ def replica_compare(ref, test, q: float = 0.9973, groups=None):
                 R, x = np.log(R), np.log(x)
         m = R.mean(axis=0)
         sd = R.std(axis=0, ddof=1)
-        ok = np.isfinite(x) & np.isfinite(m) & np.isfinite(sd) & (sd > 0)
+        # an observable that does not fluctuate in the ensemble (L, rho and
+        # betaT at constant length) differs between runs only by the
+        # rounding of its mean: constant, not compared
+        const = np.isfinite(sd) & (sd <= 1e-12 * np.abs(m))
+        ok = np.isfinite(x) & np.isfinite(m) & np.isfinite(sd) & (sd > 0) & ~const
         t = (x[ok] - m[ok]) / (sd[ok] * math.sqrt(1.0 + 1.0 / K))
         nz = ok & (m != 0)
         rd = np.abs(x[nz] - m[nz]) / np.abs(m[nz]) if nz.any() else np.zeros(1)
-        r = dict(chains_compared=int(t.size), chains_excluded=int((~ok).sum()), band=band,
+        r = dict(chains_compared=int(t.size), chains_excluded=int((~ok).sum()),
+                 chains_constant=int(const.sum()), band=band,
                  frac_within_band=float((np.abs(t) <= band).mean()) if t.size else None,
                  outside=int((np.abs(t) > band).sum()), allowed_outside=band_allowance(t.size, q),
                  mean_t=float(t.mean()) if t.size else None,

@@ -51,6 +51,21 @@
  *   orc_u32, orc_u53   end points and exact values; the trial stream
  *                      (particle pair per Philox block) recomputed from
  *                      orc_philox4x32_10 and replayed through orc_trial_with
+ *   orc_e32_pair       float32 evaluations of the LJ and Yukawa pair
+ *                      energies over hard core to cutoff lie within a
+ *                      quarter of the bound; the FP32 box-length forms (LJ
+ *                      scaling identity, Yukawa difference) within it; the
+ *                      bound far below a dropped pair term
+ *   orc_sweeps_follow  replaying the oracle's own decisions records no flip
+ *                      and ends in the same state; a flipped displacement
+ *                      or box-length decision planted in the log is recorded
+ *                      with a margin beyond its bound
+ *   orc_virials_x      three LJ particles (all pairs inside the cutoff) by
+ *                      the closed forms 48/d^12 - 24/d^6, 576/d^12 - 144/d^6
+ *   orc_sweeps_virial  the same samples and state as orc_sweeps, and after
+ *                      every sweep the virials orc_virials_x gives
+ *   schedule           the LJ exclusion distance (pot_p[2]) gives the
+ *                      engine's M_min and w (C2 at 0.5 and 0.75 sigma)
  */
 #ifndef JMM_ORACLE_H
 #define JMM_ORACLE_H

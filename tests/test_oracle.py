@@ -614,3 +614,23 @@ def test_virials_of_a_configuration_by_hand(oracle):
     ds = (1.0, 1.2, 2.2)
     assert W == pytest.approx(sum(48 / d ** 12 - 24 / d ** 6 for d in ds), rel=1e-13)
     assert HV == pytest.approx(sum(576 / d ** 12 - 144 / d ** 6 for d in ds), rel=1e-13)
+
+
+def test_sweeps_virial_is_sweeps_plus_virials(oracle):
+    """orc_sweeps_virial runs the same sweeps as orc_sweeps (the same samples
+    and final state) and records after every sweep the virials
+    orc_virials_x gives for the configuration at that point."""
+    s = oracle.make_spec(N=80, rho=0.6, T=1.0, P=0.5, ensemble=oracle.NPT, max_dv=0.02, seed=17, **lj())
+    a, b = oracle.Chain(s), oracle.Chain(s)
+    smp = a.sweeps(12)
+    vsmp, vir = b.sweeps_virial(12)
+    np.testing.assert_array_equal(smp, vsmp)
+    xa, La, _, _ = a.state()
+    xb, Lb, _, _ = b.state()
+    assert np.array_equal(xa, xb) and La == Lb
+    assert tuple(vir[-1]) == tuple(oracle.virials_x(s, xb, Lb))
+    c = oracle.Chain(s)
+    for k in range(3):
+        c.sweeps(1)
+        x, L, _, _ = c.state()
+        assert tuple(vir[k]) == tuple(oracle.virials_x(s, x, L))

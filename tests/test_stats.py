@@ -282,6 +282,13 @@ def test_replica_compare_null_and_planted_bias():
         r["L"] = np.ones(nc)
     ex = stats.replica_compare(flat, draw())
     assert ex["L"]["chains_compared"] == 0 and ex["L"]["chains_excluded"] == nc
+    # a constant whose mean over the runs carries rounding (sd ~ 1e-13)
+    # is constant, not compared
+    L0 = np.full(nc, 1024 / 0.6)
+    noisy = [dict(r, L=L0.copy()) for r in ref]
+    assert np.stack([r["L"] for r in noisy]).std(axis=0, ddof=1).max() > 0
+    cst = stats.replica_compare(noisy, dict(draw(), L=L0.copy()))
+    assert cst["L"]["chains_compared"] == 0 and cst["L"]["chains_constant"] == nc
     with pytest.raises(ValueError):
         stats.replica_compare(ref[:2], draw())
 
