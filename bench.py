@@ -240,7 +240,7 @@ def cpu_extra(chains, n_trials_per_sweep):
     algorithm (every other particle a partner of every trial, as qad2,
     src/jmmMCState.cpp:1213-1348) on one core for C1 and one C2 chain."""
     from jmmonedmc_amd import configs
-    sw1 = max(1, int(8.0 * 1.2e7 / n_trials_per_sweep))
+    sw1 = max(1, int(10.0 * 2.7e7 / n_trials_per_sweep))  # about 10 s at 2.7e7 trials/s
     v1, dt1, tr1 = cpu_baseline(chains, sw1, 1)
     out = {"single_core": {"value": v1, "unit": "trial moves/s", "cores": 1,
                            "sample": f"FP64 oracle, all {len(chains)} chains x {sw1} sweeps "
@@ -351,7 +351,7 @@ def main():
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else host
         omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
         threads = min(avail, omp) if omp > 0 else avail
-        sw = a.cpu_sweeps or max(1, int(12.0 * 1.2e7 * threads / n_trials_per_sweep))
+        sw = a.cpu_sweeps or max(1, int(15.0 * 2.4e7 * threads / n_trials_per_sweep))  # ~15 s
         v, dt, tr = leg("cpu baseline, all threads (CPU)", lambda: cpu_baseline(chains, sw, threads))
         cpu = {"value": v, "unit": "trial moves/s", "cores": threads, "kind": "port",
                "host_cpus": host, "cpus_available": avail, "omp_num_threads": omp or None,
