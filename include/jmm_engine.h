@@ -54,8 +54,9 @@ extern "C" {
                                    energy change = wavefront reduction over neighbours */
 #define JMM_FORM_BATCHED_FP64 1 /* persistent work-queue kernel, one lane per block
                                    of particles, chains packed into workgroups */
-#define JMM_FORM_BATCHED_FP32 2 /* as BATCHED_FP64 with FP32 pair arithmetic,
-                                   compensated energy change, FP64 accumulators */
+#define JMM_FORM_BATCHED_FP32 2 /* as BATCHED_FP64 with FP32 pair energies summed
+                                   in FP64; positions, cutoff decisions and
+                                   accumulators FP64 */
 
 /* One chain: the hot-path fields of the reference's MCInput
  * (src/jmmMCState.h:66-101) plus the engine's additive fields. */

@@ -248,19 +248,25 @@ def test_driver_writes_density_and_pair_histograms(gpu, oracle, driver, tmp_path
 
 
 @pytest.mark.gpu
-def test_driver_step_adaptation(gpu, driver, tmp_path):
+@pytest.mark.parametrize("mode", ["PUBLISHED", "REFERENCE"])
+def test_driver_step_adaptation(gpu, driver, tmp_path, mode):
     """DADJ/VADJ (maxDisAdjust/maxDVAdjust, src/jmmMCState.cpp:2100-2139):
     every adjustment follows Swendsen's rule from the cumulative acceptance
-    counts it prints, within the reference's clamps."""
+    counts it prints, within the reference's clamps.  ADJUST PUBLISHED (the
+    default): ln(a 0.5 + b) / ln(a P + b), box step clamped to [0.002 N,
+    0.1 N].  ADJUST REFERENCE: the reference's ln(a 0.5 + b) / ln(a (P + b))
+    (:2105, :2128) and its box step set to 0.5 N above 0.1 N (:2132-2133)."""
     inp = tmp_path / "INPUT"
     N = 60
     inp.write_text(f"ENSEMBLE NPT\nN {N}\nP 1.0\nT 0.8\nPOT LJshift 3.0\nMAXSTEP 0.05\nMAXDV 1.0\n"
                    f"NUMSTEPS {(N + 1) * 400}\nCPI {(N + 1) * 400}\nTPI {(N + 1) * 100}\n"
-                   f"DADJ {(N + 1) * 50}\nVADJ {(N + 1) * 50}\nSEED 11\nBOUNDARY WALLS\nLSTART 90\n")
-    print("[gpu] driver step adaptation", flush=True)
+                   f"DADJ {(N + 1) * 50}\nVADJ {(N + 1) * 50}\nSEED 11\nBOUNDARY WALLS\nLSTART 90\n"
+                   f"ADJUST {mode}\n")
+    print(f"[gpu] driver step adaptation {mode}", flush=True)
     r = subprocess.run([driver, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     a, b = 0.672924, 0.0644284
+    ref = mode == "REFERENCE"
     for key, start, lo, hi in (("max Step", 0.05, 0.002, 0.5), ("max vol.ch", 1.0, 0.002 * N, 0.1 * N)):
         lines = re.findall(r"Updating " + re.escape(key) + r"\.\.\.\.?\w+: (\d+),(\d+)\s+new maxStep: (\S+)",
                            r.stdout)
@@ -268,7 +274,12 @@ def test_driver_step_adaptation(gpu, driver, tmp_path):
         prev = start
         for acc, rej, new in lines:
             ratio = int(acc) / (int(acc) + int(rej))
-            want = min(hi, max(lo, prev * math.log(a * 0.5 + b) / math.log(a * ratio + b)))
+            den = math.log(a * (ratio + b)) if ref else math.log(a * ratio + b)
+            v = prev * math.log(a * 0.5 + b) / den
+            if key == "max Step":
+                want = min(hi, max(lo, v))
+            else:
+                want = lo if v < lo else (0.5 * N if ref else hi) if v > hi else v
             assert float(new) == pytest.approx(want, rel=2e-4), (key, prev, ratio, new)
             prev = float(new)
 
