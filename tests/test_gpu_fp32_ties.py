@@ -141,6 +141,19 @@ def test_c3_sample_fp32_full_length_ties(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_c2x_chain_fp32_full_length_ties(gpu, oracle):
+    """One chain of C2 with the LJ exclusion distance at 0.75 sigma (4096
+    particles, 512 blocks, the 512-lane kernel), FP32 form at the full 1e4
+    sweeps: every flip a tie within its bound, state bit for bit equal to
+    the oracle's after every chunk of 2500 sweeps."""
+    ch = configs.c2(n_chains=1, exclusion=0.75, bcap=1024)[0]
+    print("[gpu] C2x chain FP32 ties", flush=True)
+    got = run_fp32_follow(oracle, [ch], 10_000, 2_500, 500)
+    n = check_replay(got, "c2x chain", 10_000)
+    assert n <= 1e-5 * 10_000 * ch.N
+
+
+@pytest.mark.gpu
 def test_c5_chain_fp32_ties(gpu, oracle):
     """One chain of the C5 workload (65536 Yukawa particles, split into 4
     workgroups of 256 blocks), FP32 form, 200 sweeps in two launches: every

@@ -119,17 +119,23 @@ def test_c1_full_length_replays_oracle(gpu, oracle, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cfg", ["c2", "c4"])
+@pytest.mark.parametrize("cfg", ["c2", "c2x", "c4"])
 def test_config_chain_full_length_replays_oracle(gpu, oracle, cfg):
-    """One chain of C2 (4096 particles, NPT on a ring, ln L moves) or C4
-    (1024 particles, NVT at rho = 0.6) at the configuration's full 1e4
-    sweeps, batched FP64: every sweep's accepts and box length, the final
-    positions bit for bit, every counter."""
+    """One chain of C2 (4096 particles, NPT on a ring, ln L moves), of C2
+    with the LJ exclusion distance at 0.75 sigma (512 blocks, the 512-lane
+    kernel) or of C4 (1024 particles, NVT at rho = 0.6) at the
+    configuration's full 1e4 sweeps, batched FP64: every sweep's accepts and
+    box length, the final positions bit for bit, every counter."""
     from jmmonedmc_amd import configs
     sweeps, chunk = 10_000, 2_500
-    ch = getattr(configs, cfg)(n_chains=1)[0]
+    if cfg == "c2x":
+        ch = configs.c2(n_chains=1, exclusion=0.75, bcap=1024)[0]
+    else:
+        ch = getattr(configs, cfg)(n_chains=1)[0]
     print(f"[gpu] {cfg} chain full length", flush=True)
     eng = Engine([ch], form="batched_fp64", sample_capacity=chunk, block_sweeps=500)
+    if cfg == "c2x":
+        assert eng.layout(0)["B"] == 512
     parts = []
     for _ in range(sweeps // chunk):
         eng.run(chunk)
@@ -144,7 +150,7 @@ def test_config_chain_full_length_replays_oracle(gpu, oracle, cfg):
     np.testing.assert_array_equal(smp["acc_d"], osmp["acc_d"])
     np.testing.assert_array_equal(smp["acc_v"], osmp["acc_v"])
     np.testing.assert_array_equal(smp["L"], osmp["L"])
-    assert osmp["acc_v"].sum() > 0 or cfg != "c2"
+    assert osmp["acc_v"].sum() > 0 or cfg == "c4"
     assert np.array_equal(x, ox) and L == oL
     assert rel(U, oU) <= 1e-10
     for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"), ("core_rejects", "hc_rej"),
