@@ -141,6 +141,22 @@ def test_c3_sample_fp32_full_length_ties(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_c3_random_chains_fp32_ties(gpu, oracle):
+    """64 chains drawn at random (fixed seed) from the C3 workload, FP32
+    form, 2000 sweeps: every flip a tie within its bound, state bit for bit
+    equal to the oracle's after every chunk of 1000 sweeps."""
+    c3 = configs.c3()
+    pick = np.random.default_rng(41).choice(len(c3), 64, replace=False)
+    chains = [c3[i] for i in pick]
+    print(f"[gpu] C3 random 64 chains FP32 ties", flush=True)
+    got = run_fp32_follow(oracle, chains, 2000, 1000, 500)
+    n = check_replay(got, "c3 random", 2000)
+    trials = sum(c.N for c in chains) * 2000
+    print(f"[ties] c3 random: {n} flips in {trials:.3g} trials", flush=True)
+    assert n <= 1e-5 * trials
+
+
+@pytest.mark.gpu
 def test_c2x_chain_fp32_full_length_ties(gpu, oracle):
     """One chain of C2 with the LJ exclusion distance at 0.75 sigma (4096
     particles, 512 blocks, the 512-lane kernel), FP32 form at the full 1e4

@@ -202,3 +202,41 @@ def test_c3_sample_full_length_replays_oracle(gpu, oracle):
         for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"),
                       ("core_rejects", "hc_rej"), ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
             assert cnts[c][k] == orc.counter(ok), (c, k)
+
+
+@pytest.mark.gpu
+def test_c3_random_chains_replay_oracle(gpu, oracle):
+    """64 chains drawn at random (fixed seed) from the C3 workload, so the
+    whole (N, T, P) range of both potentials is sampled, in one engine for
+    2000 sweeps, batched FP64: every sweep's accepts and box length, the
+    final positions bit for bit, the energies and every counter."""
+    import concurrent.futures as cf
+    from jmmonedmc_amd import configs
+    sweeps = 2000
+    c3 = configs.c3()
+    pick = np.random.default_rng(41).choice(len(c3), 64, replace=False)
+    chains = [c3[i] for i in pick]
+    print(f"[gpu] C3 random 64 chains, {sum(c.N for c in chains)} particles", flush=True)
+    with Engine(chains, form="batched_fp64", sample_capacity=sweeps, block_sweeps=500) as eng:
+        eng.run(sweeps)
+        smp = eng.read_samples(sweeps)
+        states = [eng.state(c) for c in range(len(chains))]
+        cnts = [eng.counters(c) for c in range(len(chains))]
+
+    def replay(ch):
+        orc = oracle.Chain(oracle_spec(oracle, ch))
+        return orc, orc.sweeps(sweeps)
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        ref = list(ex.map(replay, chains))
+    for c, (orc, osmp) in enumerate(ref):
+        x, L, U, sw = states[c]
+        ox, oL, oU, osw = orc.state()
+        np.testing.assert_array_equal(smp[c]["acc_d"], osmp["acc_d"])
+        np.testing.assert_array_equal(smp[c]["acc_v"], osmp["acc_v"])
+        np.testing.assert_array_equal(smp[c]["L"], osmp["L"])
+        assert np.array_equal(x, ox) and L == oL, c
+        assert rel(U, oU) <= 1e-10
+        for k, ok in (("trials", "trials"), ("acc_d", "acc_d"), ("acc_v", "acc_v"),
+                      ("core_rejects", "hc_rej"), ("pair_terms", "pair_terms"), ("recomputes", "recomputes")):
+            assert cnts[c][k] == orc.counter(ok), (c, k)
