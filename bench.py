@@ -37,7 +37,7 @@ def parse():
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--block-sweeps", type=int, default=100)
-    p.add_argument("--workload", default="c3", choices=["c3", "c2", "c2x", "c4", "c5"])
+    p.add_argument("--workload", default="c3", choices=["c3", "c2", "c2x", "c2b", "c4", "c5"])
     p.add_argument("--plain-steps", type=int, default=1, help="sweep blocks timed for the plain FP64 form")
     p.add_argument("--forms", default="fp32,fp64,plain",
                    help="comma list of forms to time: fp32 (batched FP32, the value), fp64 "
@@ -72,6 +72,9 @@ def workload(name):
     if name == "c2x":
         return configs.c2(exclusion=0.75, bcap=1024), ("C2 with the LJ exclusion distance at 0.75 sigma "
                                                        "(0.5 sigma in C2) and 512 blocks per chain")
+    if name == "c2b":
+        return configs.c2(bcap=448), ("C2 with up to 448 blocks per chain (whole waves, 448 blocks "
+                                      "of 9-10 particles; 256 in C2)")
     if name == "c4":
         return configs.c4(), ("C4: 1024 chains, N=1024, NVT T=1, rho=0.6, shifted LJ rc=3, "
                               "(eps, sigma) = (1, 1); after every sweep block the backward "

@@ -79,7 +79,11 @@ typedef struct jmm_chain_spec {
     uint64_t seed;      /* reference SEED: Philox4x32-10 key */
     uint32_t chain_id;  /* Philox counter word: distinct streams per seed */
     uint32_t eci;       /* reference ENGCHECK, in sweeps: FP64 recompute interval, 0 = never */
-    uint32_t bcap;      /* largest number of particle blocks (power of two, 0 = 256) */
+    uint32_t bcap;      /* largest number of particle blocks, at most 1024 (0 = 256).  A
+                           power of two gives power-of-two block counts; a multiple
+                           of 64 that is not one gives block counts that are
+                           multiples of 64 (whole waves) when at least 64; such a
+                           chain is never split over workgroups */
     uint32_t reserved;
 } jmm_chain_spec;
 

@@ -74,7 +74,9 @@ double gmin_of(const jmm_chain_spec &s)
 // (rc + delta)/g_min places away); M_min = smallest index separation of two
 // trials that can run at once (their partner sets and moved particles must
 // stay apart: (rc + 2 delta)/g_min, or NBN + 1); B = power-of-two number of
-// blocks with every block at least M_min long, capped by bcap.
+// blocks with every block at least M_min long, capped by bcap; a bcap that is
+// not a power of two (a multiple of 64) allows B to be any multiple of 64
+// up to it (whole waves per chain), or a power of two below 64.
 void schedule_of(const jmm_chain_spec &s, uint64_t &B, uint64_t &M, uint64_t &w)
 {
     const uint64_t N = s.N;
@@ -98,7 +100,8 @@ void schedule_of(const jmm_chain_spec &s, uint64_t &B, uint64_t &M, uint64_t &w)
         uint64_t q = N / mc;
         const uint64_t cap = s.bcap ? s.bcap : 256;
         if (q > cap) q = cap;
-        while (nb * 2 <= q) nb *= 2;
+        if ((cap & (cap - 1)) != 0 && q >= 64) nb = q / 64 * 64;  // whole waves
+        else while (nb * 2 <= q) nb *= 2;
     }
     B = nb;
     M = mc;
@@ -115,6 +118,8 @@ int validate(const jmm_chain_spec &s, int idx)
     if (s.ensemble != JMM_NLT && s.ensemble != JMM_NPT) return fail(JMM_EINVAL, "chain %d: bad ensemble", idx);
     if (s.vol_move != JMM_VOL_LINEAR && s.vol_move != JMM_VOL_LOG) return fail(JMM_EINVAL, "chain %d: bad vol_move", idx);
     if (!(s.cutoff > 0.0)) return fail(JMM_EINVAL, "chain %d: cutoff must be > 0", idx);
+    if (s.bcap > 1024 || ((s.bcap & (s.bcap - 1)) != 0 && s.bcap % 64 != 0))
+        return fail(JMM_EINVAL, "chain %d: bcap must be a power of two or a multiple of 64, at most 1024", idx);
     if (!(s.L0 > 0.0) || !std::isfinite(s.L0)) return fail(JMM_EINVAL, "chain %d: L0 must be finite and > 0", idx);
     if (s.potential == JMM_POT_LJ && !(s.pot_p[1] > 0.0)) return fail(JMM_EINVAL, "chain %d: LJ sigma must be > 0", idx);
     if (s.potential == JMM_POT_HRSW) {
@@ -625,6 +630,9 @@ int jmm_create(const jmm_chain_spec *specs, int n_chains, const jmm_engine_opts 
         const size_t thr = o.seg_lds ? std::min<size_t>(o.seg_lds / 8, dev_max) : dev_max;
         for (int c = 0; c < n_chains; ++c) {
             if ((size_t)ch[c].lds_len <= thr) continue;
+            // segments are power-of-two shares of the blocks: a chain whose
+            // block count is not a power of two stays whole (LDS or HBM)
+            if ((ch[c].B & (ch[c].B - 1)) != 0) continue;
             for (uint32_t S = 2; S <= ch[c].B; S *= 2) {
                 const uint32_t Bs = ch[c].B / S;
                 const size_t img = 2 * (size_t)ch[c].wg + (size_t)Bs * ch[c].Sp;

@@ -276,13 +276,15 @@ __device__ __noinline__ VolProp vol_propose(const DevChain &ch, double L, uint64
 }
 
 
-// Sum over aligned power-of-two lane segments of size seg (seg <= T).  Every
-// lane of a segment receives the same bits: the butterfly is symmetric in its
-// operands, and the cross-wave stage reads the wave totals in a fixed order.
+// Sum over lane segments of size seg (seg <= T) whose first lane is base:
+// a power of two up to 64 with base a multiple of seg, or a multiple of 64
+// with base a multiple of 64 (whole waves).  Every lane of a segment
+// receives the same bits: the butterfly is symmetric in its operands, and
+// the cross-wave stage reads the segment's wave totals in a fixed order.
 // rd/ri: flat LDS scratch of (T/64)*ND doubles and (T/64)*NI words.
 template <int T, int ND, int NI>
 __device__ __forceinline__ void seg_allreduce(double (&v)[ND], uint32_t (&k)[NI], uint32_t seg,
-                                              double *rd, uint32_t *ri)
+                                              uint32_t base, double *rd, uint32_t *ri)
 {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -307,7 +309,7 @@ __device__ __forceinline__ void seg_allreduce(double (&v)[ND], uint32_t (&k)[NI]
         }
         __syncthreads();
         if (seg > 64) {
-            const uint32_t w0 = (tid & ~(seg - 1)) >> 6, nw = seg >> 6;
+            const uint32_t w0 = base >> 6, nw = seg >> 6;
 #pragma unroll
             for (int m = 0; m < ND; ++m) {
                 double acc = 0.0;
@@ -464,7 +466,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
         {
             double v[3] = {dUs, vpart, upart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<T, 3, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 3, 2>(v, k2, seg, me.lane_off, sc.rd, sc.ri);
             U = U + v[0];
             vpart = v[1];
             acc = k2[0];
@@ -497,7 +499,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
             double v[1] = {need ? energy_part<KIND>(q, X, N, per, L, w, start, start + size, 1)
                                 : 0.0};
             uint32_t k1[1] = {0};
-            seg_allreduce<T, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 1, 1>(v, k1, seg, me.lane_off, sc.rd, sc.ri);
             if (need) {
                 const double dr = fabs(v[0] - U);
                 if (dr > mdrift) mdrift = dr;
@@ -511,7 +513,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
             double v[2] = {0.0, 0.0};
             if (act) virial_part<KIND>(q, X, N, per, L, w, start, start + size, 1, v[0], v[1]);
             uint32_t k1[1] = {0};
-            seg_allreduce<T, 2, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 2, 1>(v, k1, seg, me.lane_off, sc.rd, sc.ri);
             Ws = v[0];
             Hs = v[1];
         }
@@ -530,7 +532,7 @@ __device__ __forceinline__ void run_bundle_gx(const SweepArgs &a, const DevBundl
     {
         double vz[1] = {0.0};
         uint32_t k4[4] = {n_tr, n_wall, n_core, n_terms};
-        seg_allreduce<T, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        seg_allreduce<T, 1, 4>(vz, k4, seg, me.lane_off, sc.rd, sc.ri);
         if (leader) {
             DevState &o = a.states[ic];
             o.L = L;
@@ -1119,7 +1121,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         {
             double v[3] = {dUs, vpart, upart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<T, 3, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 3, 2>(v, k2, seg, me.lane_off, sc.rd, sc.ri);
             U = U + v[0];
             vpart = v[1];
             acc = k2[0];
@@ -1169,7 +1171,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
             }
             double v[1] = {Up};
             uint32_t k1[1] = {0};
-            seg_allreduce<T, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 1, 1>(v, k1, seg, me.lane_off, sc.rd, sc.ri);
             if (need) {
                 const double dr = fabs(v[0] - U);
                 if (dr > mdrift) mdrift = dr;
@@ -1183,7 +1185,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
             double v[2] = {0.0, 0.0};
             if (act) virial_lane<KIND>(c, q, start, size, a0, gapR, v[0], v[1]);
             uint32_t k1[1] = {0};
-            seg_allreduce<T, 2, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<T, 2, 1>(v, k1, seg, me.lane_off, sc.rd, sc.ri);
             Ws = v[0];
             Hs = v[1];
         }
@@ -1202,7 +1204,7 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
     {
         double vz[1] = {0.0};
         uint32_t k4[4] = {n_tr, n_wall, n_core, n_terms};
-        seg_allreduce<T, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        seg_allreduce<T, 1, 4>(vz, k4, seg, me.lane_off, sc.rd, sc.ri);
         if (leader) {
             DevState &o = a.states[ic];
             o.L = L;
@@ -1555,7 +1557,7 @@ __device__ __forceinline__ void run_segment(const SegArgs &g, const DevSeg &sg, 
         {
             double v[3] = {dUs, vpart, upart};
             uint32_t k2[2] = {acc, bad};
-            seg_allreduce<TW, 3, 2>(v, k2, seg, sc.rd, sc.ri);
+            seg_allreduce<TW, 3, 2>(v, k2, seg, 0u, sc.rd, sc.ri);
             if (tid == 0) {
                 put(par, 0, 0, 3);
                 put(par, 1, 1, 3);
@@ -1613,7 +1615,7 @@ __device__ __forceinline__ void run_segment(const SegArgs &g, const DevSeg &sg, 
             }
             double v[1] = {Up};
             uint32_t k1[1] = {0};
-            seg_allreduce<TW, 1, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<TW, 1, 1>(v, k1, seg, 0u, sc.rd, sc.ri);
             if (tid == 0) put(par, 4, 0, 1);
             sync_all();
             gather(par, 4);
@@ -1628,7 +1630,7 @@ __device__ __forceinline__ void run_segment(const SegArgs &g, const DevSeg &sg, 
             double v[2] = {0.0, 0.0};
             if (act && own) virial_lane<KIND>(c, q, start, size, a0, gapR, v[0], v[1]);
             uint32_t k1[1] = {0};
-            seg_allreduce<TW, 2, 1>(v, k1, seg, sc.rd, sc.ri);
+            seg_allreduce<TW, 2, 1>(v, k1, seg, 0u, sc.rd, sc.ri);
             if (tid == 0) {
                 put(par, 6, 0, 2);
                 put(par, 7, 1, 2);
@@ -1656,7 +1658,7 @@ __device__ __forceinline__ void run_segment(const SegArgs &g, const DevSeg &sg, 
     {
         double vz[1] = {0.0};
         uint32_t k4[4] = {own ? n_tr : 0u, own ? n_wall : 0u, own ? n_core : 0u, n_terms};
-        seg_allreduce<TW, 1, 4>(vz, k4, seg, sc.rd, sc.ri);
+        seg_allreduce<TW, 1, 4>(vz, k4, seg, 0u, sc.rd, sc.ri);
         const uint32_t par = (uint32_t)(sweep & 1u);
         if (tid == 0)
             for (uint32_t m = 0; m < 4; ++m) g.part[seg_pidx(me, par, 5, m)] = (double)k4[m];

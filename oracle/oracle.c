@@ -274,9 +274,10 @@ static double gmin_of(const orc_spec *s)
     }
 }
 
-/* B blocks (a power of two), M_min = minimum index separation of trials
- * that may run at once, w = largest index distance of any partner a trial
- * can interact with. */
+/* B blocks (a power of two, or with a bcap that is not a power of two a
+ * multiple of 64 when at least 64), M_min = minimum index separation of
+ * trials that may run at once, w = largest index distance of any partner a
+ * trial can interact with. */
 void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w)
 {
     uint64_t N = s->N;
@@ -300,7 +301,8 @@ void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w)
         uint64_t q = N / mc;
         uint64_t cap = s->bcap ? s->bcap : 256;
         if (q > cap) q = cap;
-        while (nb * 2 <= q) nb *= 2;
+        if ((cap & (cap - 1)) != 0 && q >= 64) nb = q / 64 * 64; /* whole waves */
+        else while (nb * 2 <= q) nb *= 2;
     }
     *B = nb;
     *M_min = mc;

@@ -96,7 +96,8 @@ typedef struct orc_spec {
     uint64_t seed;
     uint32_t chain_id; /* Philox stream id */
     uint32_t eci;      /* full-energy recompute interval in sweeps, 0 = never */
-    uint32_t bcap;     /* maximum number of blocks (power of two) */
+    uint32_t bcap;     /* maximum number of blocks (a power of two; otherwise a
+                          multiple of 64, and block counts are multiples of 64) */
     uint32_t pad;
 } orc_spec;
 

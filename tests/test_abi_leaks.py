@@ -26,7 +26,8 @@ OURS = ("libjmm_engine_asan", "engine.hip", "kernels.hip", "abi_lifecycle")
 def run(mode, timeout):
     assert os.path.exists(PROG), f"{PROG} missing: build with __graft_entry__.build()"
     env = dict(os.environ)
-    env["ASAN_OPTIONS"] = "detect_leaks=1:verify_asan_link_order=0"
+    # exitcode=23: the status a leak report ends the program with
+    env["ASAN_OPTIONS"] = "detect_leaks=1:verify_asan_link_order=0:exitcode=23"
     env["LSAN_OPTIONS"] = f"suppressions={SUPP}:print_suppressions=0"
     sym = "/opt/rocm/llvm/bin/llvm-symbolizer"
     if os.path.exists(sym):
@@ -48,6 +49,7 @@ def check(r, mode):
     assert f"{mode}: 0 failed checks" in r.stdout, out[-6000:]
     # 23: LeakSanitizer found leaks, all of them outside the engine (above)
     assert r.returncode in (0, 23), (r.returncode, out[-6000:])
+    assert "CHECK failed" not in out, out[-6000:]
 
 
 def test_error_paths_leak_free():

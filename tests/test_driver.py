@@ -143,6 +143,29 @@ def test_lj_exclusion_distance_schedule(oracle):
     assert b"exclusion" in lib.jmm_last_error()
 
 
+def test_block_counts_in_whole_waves(oracle):
+    """A bcap that is not a power of two (a multiple of 64) gives block
+    counts that are multiples of 64 (floor(N / M_min / 64) * 64, at most
+    bcap), or a power of two below 64; engine and oracle alike.  Other
+    bcap values are refused."""
+    import ctypes as C
+    from jmmonedmc_amd import _native
+    from jmmonedmc_amd.engine import Chain, schedule
+    from helpers import lj, oracle_spec
+    for N, bcap, B in ((4096, 448, 448), (4096, 0, 256), (1800, 448, 192), (900, 448, 64), (300, 448, 32),
+                       (8192, 448, 448), (4096, 512, 256), (6000, 960, 640)):
+        ch = Chain(N=N, rho=0.5, bcap=bcap, **lj())
+        got = tuple(schedule(ch))
+        assert got[0] == B and got == tuple(oracle.schedule(oracle_spec(oracle, ch))), (N, bcap, got)
+        assert N // got[0] >= got[1]
+    lib = _native.lib()
+    for bad in (100, 2048):
+        s = Chain(N=64, L0=128.0, bcap=bad, **lj()).to_c()
+        h = C.c_void_p()
+        assert lib.jmm_create(C.byref(s), 1, None, C.byref(h)) != 0
+        assert b"bcap" in lib.jmm_last_error()
+
+
 @pytest.mark.gpu
 def test_driver_replays_oracle(gpu, oracle, driver, tmp_path):
     """The driver on a C1-shaped INPUT (periodic, shifted LJ, NVT): the final
