@@ -231,3 +231,35 @@ def test_every_placement_in_one_engine_replays_oracle(gpu, oracle):
         assert np.array_equal(x, ox) and L == oL and sw == osw == 4, c
         assert rel(U, oU) <= 1e-10, c
         assert np.array_equal(smp[c]["acc_v"], osmp["acc_v"]), c
+
+
+def test_whole_wave_chains_stay_whole(gpu, oracle):
+    """Chains whose block count is a multiple of 64 but not a power of two
+    (bcap 448) are never split: one too large for a workgroup's LDS runs
+    from HBM (40000 particles, 448 blocks), one that a small split
+    threshold would split otherwise stays in LDS (4096 particles); FP64
+    form, each equal to the oracle move for move."""
+    big = Chain(N=40000, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT, vol_move=nat.VOL_LOG,
+                max_dv=0.01, seed=41, chain_id=41, bcap=448, **lj())
+    mid = Chain(N=4096, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT, vol_move=nat.VOL_LOG,
+                max_dv=0.01, seed=42, chain_id=42, bcap=448, **lj())
+    chains = [big, mid]
+    print("[gpu] whole-wave chains: hbm + lds under a 16 KiB split threshold", flush=True)
+    with Engine(chains, form="batched_fp64", sample_capacity=4, block_sweeps=2, seg_lds=16384,
+                log_accepts=True) as e:
+        lay = [e.layout(c) for c in range(2)]
+        assert [l["mode"] for l in lay] == ["hbm", "lds"], lay
+        assert [l["B"] for l in lay] == [448, 448], lay
+        e.run(4)
+        logs = e.read_accept_log(4)
+        smp = e.read_samples(4)
+        states = [e.state(c) for c in range(2)]
+    for c, ch in enumerate(chains):
+        orc = oracle.Chain(oracle_spec(oracle, ch))
+        osmp, olog = orc.sweeps(4, log=True)
+        ox, oL, oU, osw = orc.state()
+        x, L, U, sw = states[c]
+        assert np.array_equal(logs[c].ravel(), olog), c
+        assert np.array_equal(x, ox) and L == oL and sw == osw == 4, c
+        assert rel(U, oU) <= 1e-10, c
+        assert np.array_equal(smp[c]["acc_v"], osmp["acc_v"]), c
