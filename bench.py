@@ -68,13 +68,14 @@ def workload(name):
                               "1/2 shifted LJ rc=3 + 1/2 hard rod 1 + square well 1.5, NPT ln L "
                               "+-0.01, step +-0.5, density 0.5 start, FP64 recompute every 100 sweeps")
     if name == "c2":
-        return configs.c2(), "C2: 256 chains, N=4096, NPT T=1 P=0.5, shifted LJ rc=3"
+        return configs.c2(), ("C2: 256 chains, N=4096, NPT T=1 P=0.5, shifted LJ rc=3 "
+                              "(512 blocks of 8 per chain)")
     if name == "c2x":
-        return configs.c2(exclusion=0.75, bcap=1024), ("C2 with the LJ exclusion distance at 0.75 sigma "
-                                                       "(0.5 sigma in C2) and 512 blocks per chain")
+        return configs.c2(exclusion=0.75), ("C2 with the LJ exclusion distance at 0.75 sigma "
+                                            "(0.5 sigma in C2): a window of 4 partners instead of 6")
     if name == "c2b":
-        return configs.c2(bcap=448), ("C2 with up to 448 blocks per chain (whole waves, 448 blocks "
-                                      "of 9-10 particles; 256 in C2)")
+        return configs.c2(bcap=448), ("C2 with block counts in whole waves up to 448 "
+                                      "(448 blocks of 9-10 particles per chain; 512 in C2)")
     if name == "c4":
         return configs.c4(), ("C4: 1024 chains, N=1024, NVT T=1, rho=0.6, shifted LJ rc=3, "
                               "(eps, sigma) = (1, 1); after every sweep block the backward "

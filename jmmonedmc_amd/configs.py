@@ -30,14 +30,15 @@ def c1(boundary=nat.PERIODIC, seed=1):
                   max_step=0.5, **LJ)]
 
 
-def c2(n_chains=256, N=4096, exclusion=0.0, bcap=0):
-    """256 NPT LJ chains of 4096.  exclusion > 0: the LJ exclusion distance
-    (a trial closer than it is rejected; the default is 0.5 sigma) and bcap
-    the largest block count; exclusion 0.75 with bcap 1024 gives 512 blocks
-    per chain instead of 256 (M_min = floor((3 + 2 * 0.5) / 0.75) + 1 = 6);
-    trials landing closer than 0.75 sigma, whose pair energy is at least
-    phi(0.75) = 103.8 eps (Boltzmann factor 8e-46 at T = 1), are then
-    rejected without evaluation."""
+def c2(n_chains=256, N=4096, exclusion=0.0, bcap=1024):
+    """256 NPT LJ chains of 4096.  bcap: the largest block count (1024:
+    4096 / M_min = 4096 / 7 = 585 rounds down to 512 blocks per chain, the
+    512-lane kernel; 0 caps at 256).  exclusion > 0: the LJ exclusion
+    distance (a trial closer than it is rejected; the default is 0.5 sigma);
+    at 0.75 sigma the window is 4 partners instead of 6, and trials landing
+    closer than 0.75 sigma, whose pair energy is at least phi(0.75) =
+    103.8 eps (Boltzmann factor 8e-46 at T = 1), are rejected without
+    evaluation."""
     lj = dict(LJ, pot_p=(LJ["pot_p"][0], LJ["pot_p"][1], exclusion, 0.0))
     return [Chain(N=N, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT, vol_move=nat.VOL_LOG, max_dv=0.01,
                   max_step=0.5, seed=s, chain_id=s, bcap=bcap, **lj) for s in range(n_chains)]

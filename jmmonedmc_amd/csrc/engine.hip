@@ -68,15 +68,25 @@ double gmin_of(const jmm_chain_spec &s)
     }
 }
 
-// The trial schedule (also restated by the oracle): w = largest index distance
-// of a partner that can interact with a trial (hard core g_min keeps every gap
-// >= g_min, so a partner within rc of the old or new position is at most
-// (rc + delta)/g_min places away); M_min = smallest index separation of two
-// trials that can run at once (their partner sets and moved particles must
-// stay apart: (rc + 2 delta)/g_min, or NBN + 1); B = power-of-two number of
-// blocks with every block at least M_min long, capped by bcap; a bcap that is
-// not a power of two (a multiple of 64) allows B to be any multiple of 64
-// up to it (whole waves per chain), or a power of two below 64.
+// The trial schedule (also restated by the oracle).  Every state keeps each
+// gap between neighbours (and across the periodic wrap) at least g_min: the
+// lattice and set_state are checked, a displacement that would bring its
+// particle within g_min of either neighbour is rejected before any energy
+// work, and a length move that would do so is rejected.  So partner k of a
+// particle is at least k g_min away, from its old position and from any new
+// position that passes the hard-core test (the neighbour's gap adds to the
+// k - 1 gaps beyond it); the same holds for the scaled configuration of a
+// length move that is not rejected for an overlap.  w = largest index
+// distance of a partner that can lie within rc: floor(rc / g_min), taken
+// with a relative guard of 1e-3 so that partner w + 1 is beyond rc by at
+// least 1e-3 g_min, far more than the rounding of FP64 separations (or NBN,
+// or N - 1).  M_min = w + 1 (at least 2, as the hard-core test reads both
+// nearest partners): two trials at least M_min indices apart read none of
+// each other's particles and, in every order, stay beyond rc of each other,
+// so they commute.  B = power-of-two number of blocks with every block at
+// least M_min long, capped by bcap; a bcap that is not a power of two (a
+// multiple of 64) allows B to be any multiple of 64 up to it (whole waves
+// per chain), or a power of two below 64.
 void schedule_of(const jmm_chain_spec &s, uint64_t &B, uint64_t &M, uint64_t &w)
 {
     const uint64_t N = s.N;
@@ -84,10 +94,9 @@ void schedule_of(const jmm_chain_spec &s, uint64_t &B, uint64_t &M, uint64_t &w)
     uint64_t wc = big, mc = big + 1;
     const double g = gmin_of(s);
     if (std::isfinite(s.cutoff) && g > 1e-300) {
-        const double a = std::floor((s.cutoff + s.max_step) / g);
-        const double b = std::floor((s.cutoff + 2.0 * s.max_step) / g) + 1.0;
+        const double a = std::floor(s.cutoff / g + 1e-3);
         if (a < (double)big) wc = (uint64_t)a;
-        if (b < (double)(big + 1)) mc = (uint64_t)b;
+        if (a + 1.0 < (double)(big + 1)) mc = (uint64_t)a + 1;
     }
     if (s.nbn >= 0) {
         if ((uint64_t)s.nbn < wc) wc = (uint64_t)s.nbn;
@@ -1474,6 +1483,9 @@ int jmm_pair_observables(const jmm_chain_spec *spec, int n_configs, const double
         return fail(JMM_EINVAL, "jmm_pair_observables: bad potential description");
     ObsArgs a{};
     a.one = make_chain(s);
+    // configurations from the caller may hold gaps below the hard core: walk
+    // every partner to the cutoff (or the NBN limit), not the chain's window
+    a.one.w = std::min<uint64_t>(a.one.nbn_lim, s.N - 1);
     a.x = x_dev;
     a.L = L_dev;
     a.obs = obs_dev;

@@ -1038,6 +1038,8 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
     c.rem = ch.N % me.B;
     c.per = (sh.flags & F_PERIODIC) != 0;
     const bool npt = (sh.flags & F_NPT) != 0;
+    // FP32: walks of one partner per step where the window is one partner
+    const bool one = sizeof(Real) == 4 && c.w == 1;
 
     const uint32_t r = tid - me.lane_off;
     const uint32_t size = act ? c.qb + (r < c.rem ? 1u : 0u) : 0u;
@@ -1091,9 +1093,15 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
                 const double xn = xi + md;
                 ++n_tr;
                 double dE = 0.0;
-                const bool accd = trial_lds<Real, KIND, false, true>(c, q, gmin, beta, betaf, i, t, size,
-                                                                     my, gapL, gapR, L, xi, xn, md, u, dE,
-                                                                     n_terms, n_wall, n_core);
+                // a window of one partner per side (the hard core keeps the
+                // second beyond the cutoff): the one-partner walk, which
+                // evaluates no clamped duplicate (wave-uniform: a bundle
+                // shares its window)
+                const bool accd =
+                    one ? trial_lds<Real, KIND, false, false>(c, q, gmin, beta, betaf, i, t, size, my, gapL, gapR,
+                                                              L, xi, xn, md, u, dE, n_terms, n_wall, n_core)
+                        : trial_lds<Real, KIND, false, true>(c, q, gmin, beta, betaf, i, t, size, my, gapL, gapR,
+                                                             L, xi, xn, md, u, dE, n_terms, n_wall, n_core);
                 const double xs = accd ? xn : xi;
                 c.R[my] = xs;
                 if (c.per) {
@@ -1114,9 +1122,12 @@ __device__ __forceinline__ void run_bundle_lds(const SweepArgs &a, const DevBund
         double vpart = 0.0, upart = 0.0;
         if (npt) {
             vp = vol_propose(ch, L, sweep);
-            vpart = vol_lane<Real, KIND, sizeof(Real) == 4>(
-                c, q, gmin, vp, vp.valid && act, start, size, a0, gapR, c.per && r + 1 == c.B,
-                [&](uint32_t m) { return c.R[c.wg + m]; }, bad, upart);
+            auto wrap = [&](uint32_t m) { return c.R[c.wg + m]; };
+            const bool last = c.per && r + 1 == c.B, von = vp.valid && act;
+            vpart = one ? vol_lane<Real, KIND, false>(c, q, gmin, vp, von, start, size, a0, gapR, last, wrap, bad,
+                                                      upart)
+                        : vol_lane<Real, KIND, sizeof(Real) == 4>(c, q, gmin, vp, von, start, size, a0, gapR, last,
+                                                                  wrap, bad, upart);
         }
         {
             double v[3] = {dUs, vpart, upart};

@@ -277,7 +277,12 @@ static double gmin_of(const orc_spec *s)
 /* B blocks (a power of two, or with a bcap that is not a power of two a
  * multiple of 64 when at least 64), M_min = minimum index separation of
  * trials that may run at once, w = largest index distance of any partner a
- * trial can interact with. */
+ * trial can interact with.  Every gap between neighbours stays at least
+ * g_min (the hard-core test rejects a displacement or length move that
+ * would break it), so partner k is at least k g_min from the old position
+ * and from any new position that passes that test: w = floor(rc / g_min)
+ * with a relative guard of 1e-3 (partner w + 1 lies beyond rc by at least
+ * 1e-3 g_min), M_min = w + 1 (at least 2). */
 void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w)
 {
     uint64_t N = s->N;
@@ -285,10 +290,9 @@ void orc_schedule(const orc_spec *s, uint64_t *B, uint64_t *M_min, uint64_t *w)
     uint64_t wc = big, mc = big + 1;
     double g = gmin_of(s);
     if (isfinite(s->cutoff) && g > 1e-300) {
-        double a = floor((s->cutoff + s->max_step) / g);
-        double b = floor((s->cutoff + 2.0 * s->max_step) / g) + 1.0;
+        double a = floor(s->cutoff / g + 1e-3);
         if (a < (double)big) wc = (uint64_t)a;
-        if (b < (double)(big + 1)) mc = (uint64_t)b;
+        if (a + 1.0 < (double)(big + 1)) mc = (uint64_t)a + 1;
     }
     if (s->nbn >= 0) {
         if ((uint64_t)s->nbn < wc) wc = (uint64_t)s->nbn;
@@ -941,7 +945,9 @@ static int trial_allpairs(orc_chain *c, uint64_t i, double md, double u)
             if (s->nbn < 0 || kr <= s->nbn)
                 dE += phi_e(s, xjr - xn, c->shift) - phi_e(s, xjr - xi, c->shift);
             int64_t kl = N - kr;
-            double xjl = xjr - c->L;
+            /* the left image: x_j itself for j < i, x_j - L for j > i (the
+             * window's images; xjr - L would round differently) */
+            double xjl = j > (int64_t)i ? c->x[j] - c->L : c->x[j];
             if (s->nbn < 0 || kl <= s->nbn)
                 dE += phi_e(s, xn - xjl, c->shift) - phi_e(s, xi - xjl, c->shift);
         }

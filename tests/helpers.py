@@ -53,11 +53,10 @@ CASES = {
                                        vol_move=nat.VOL_LOG, max_dv=0.01, seed=8, chain_id=4, bcap=1024,
                                        **lj(pot_p=(1.0, 1.0, 0.75, 0.0)))],
     # block counts in whole waves (bcap 448, not a power of two): LJ NPT
-    # chains of 4096 (448 blocks), 1800 (192) and 300 (32, below a wave) in
-    # one 512-lane workgroup
+    # chains of 4096 (448 blocks), 1400 (192) and 300 (32, below a wave)
     "blocks64_npt": lambda: [Chain(N=n, rho=0.5, T=1.0, P=0.5, ensemble=nat.NPT, vol_move=nat.VOL_LOG,
                                    max_dv=0.01, seed=60 + k, chain_id=60 + k, bcap=448, **lj())
-                             for k, n in enumerate([4096, 1800, 300])],
+                             for k, n in enumerate([4096, 1400, 300])],
     # hard rod + square well, NPT periodic (C3 second half)
     "hrsw_npt": lambda: [Chain(N=200, rho=0.5, T=1.0, P=1.0, ensemble=nat.NPT, max_dv=0.01,
                                seed=5, **hrsw())],

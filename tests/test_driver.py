@@ -123,19 +123,33 @@ def test_schedule_and_validation_without_gpu():
 
 
 def test_lj_exclusion_distance_schedule(oracle):
-    """The LJ exclusion distance sets the schedule in the engine and the
-    oracle alike: C2's chain at 0.5 sigma (default) has M_min = 9 and 256
-    blocks; at 0.75 sigma M_min = floor(4 / 0.75) + 1 = 6, w = floor(3.5 /
-    0.75) = 4, and with bcap 1024 the 4096 particles take 512 blocks.  An
+    """The schedule from the hard core alone, engine and oracle alike:
+    w = floor(rc / g + 1e-3), M_min = w + 1.  C2's chain at the default LJ
+    exclusion 0.5 sigma: w = 6, M_min = 7, 256 blocks (the default cap) or
+    512 with bcap 1024; at 0.75 sigma w = 4, M_min = 5.  Hard rod + square
+    well (g = 1, rc = 1.5): w = 1, M_min = 2; C5's Yukawa (g = 0.5, rc = 20):
+    w = 40, M_min = 41, 1024 blocks.  rc / g just below an integer (within
+    the guard) counts as that integer; further below, one partner less.  An
     exclusion outside the cutoff is refused."""
     import ctypes as C
     from jmmonedmc_amd import _native
-    from jmmonedmc_amd.configs import c2
+    from jmmonedmc_amd.configs import c2, c3_params, c5
     from jmmonedmc_amd.engine import Chain, schedule
-    from helpers import oracle_spec
-    base, wide = c2(1)[0], c2(1, exclusion=0.75, bcap=1024)[0]
-    assert tuple(schedule(base)) == (256, 9, 7) == tuple(oracle.schedule(oracle_spec(oracle, base)))
-    assert tuple(schedule(wide)) == (512, 6, 4) == tuple(oracle.schedule(oracle_spec(oracle, wide)))
+    from helpers import hrsw, lj, oracle_spec
+
+    def both(ch):
+        got = tuple(schedule(ch))
+        assert got == tuple(oracle.schedule(oracle_spec(oracle, ch))), got
+        return got
+
+    assert both(c2(1, bcap=0)[0]) == (256, 7, 6)
+    assert both(c2(1)[0]) == (512, 7, 6)
+    assert both(c2(1, exclusion=0.75)[0]) == (512, 5, 4)
+    assert both(Chain(N=4096, rho=0.5, **hrsw())) == (256, 2, 1)
+    assert both(Chain(N=40, rho=0.5, **hrsw())) == (16, 2, 1)
+    assert both(c5(1)[0]) == (1024, 41, 40)
+    for gap, w in ((0.5, 6), (0.5 + 4e-5, 6), (0.5 + 0.01, 5)):
+        assert both(Chain(N=4096, rho=0.5, cutoff=3.0, pot_p=(1.0, 1.0, gap, 0.0))) [1:] == (w + 1, w)
     lib = _native.lib()
     s = Chain(N=10, L0=40.0, cutoff=3.0, pot_p=(1.0, 1.0, 3.5, 0.0)).to_c()
     h = C.c_void_p()
@@ -152,8 +166,8 @@ def test_block_counts_in_whole_waves(oracle):
     from jmmonedmc_amd import _native
     from jmmonedmc_amd.engine import Chain, schedule
     from helpers import lj, oracle_spec
-    for N, bcap, B in ((4096, 448, 448), (4096, 0, 256), (1800, 448, 192), (900, 448, 64), (300, 448, 32),
-                       (8192, 448, 448), (4096, 512, 256), (6000, 960, 640)):
+    for N, bcap, B in ((4096, 448, 448), (4096, 0, 256), (1400, 448, 192), (2500, 448, 320), (600, 448, 64),
+                       (300, 448, 32), (8192, 448, 448), (4096, 512, 512), (6000, 960, 832)):
         ch = Chain(N=N, rho=0.5, bcap=bcap, **lj())
         got = tuple(schedule(ch))
         assert got[0] == B and got == tuple(oracle.schedule(oracle_spec(oracle, ch))), (N, bcap, got)

@@ -196,8 +196,10 @@ def test_schedule_invariants(oracle, name):
         if ch.nbn >= 0:
             assert w <= ch.nbn and M <= ch.nbn + 1
         if g > 0 and math.isfinite(ch.cutoff) and ch.nbn < 0 and M <= N:
-            assert M * g > ch.cutoff + 2 * ch.max_step
-            assert w >= min(N - 1, math.floor((ch.cutoff + ch.max_step) / g))
+            # partner M (and w + 1) lies beyond rc by at least the guard,
+            # every partner that can lie within rc is walked
+            assert M == w + 1 and M * g >= ch.cutoff + 1e-3 * g
+            assert w >= min(N - 1, math.floor(ch.cutoff / g))
         # engine and oracle compute the same schedule
         from jmmonedmc_amd.engine import schedule
         assert schedule(ch) == (B, M, w)
@@ -273,10 +275,13 @@ def test_window_energy_equals_brute_force(oracle, kw, gmin, boundary, nbn):
         assert rel(uw, ub) <= 1e-13, (N, uw, ub)
 
 
+@pytest.mark.parametrize("reach", [1.0, 4.0])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_trial_energy_change_equals_brute_force(oracle, name):
+def test_trial_energy_change_equals_brute_force(oracle, name, reach):
     """dE of single trials equals brute(after) - brute(before) on every
-    accepted trial, and rejected trials leave the state untouched."""
+    accepted trial, and rejected trials leave the state untouched; also
+    for displacements up to four times the chain's step (the partner window
+    rests on the hard core alone, not on the step size)."""
     import pyoracle
     ch = CASES[name]()[0]
     s = oracle_spec(pyoracle, ch)
@@ -284,7 +289,7 @@ def test_trial_energy_change_equals_brute_force(oracle, name):
     rng = np.random.default_rng(2)
     for _ in range(300):
         i = int(rng.integers(ch.N))
-        md = float(rng.uniform(-ch.max_step, ch.max_step))
+        md = float(rng.uniform(-reach * ch.max_step, reach * ch.max_step))
         x0, L, U0, _ = c.state()
         b0 = c.brute_energy()
         acc, dE = c.trial_with(i, md, 0.0)   # u = 0: every finite dE is accepted
@@ -418,7 +423,8 @@ def test_incremental_energy_equals_recompute(oracle, name):
 
 
 @pytest.mark.parametrize("name", ["c1_walls_truncated", "npt_walls_linear", "lj_inf_walls_npt",
-                                  "harmonic_nbn1_walls", "hrsw_npt", "npt_periodic_log"])
+                                  "harmonic_nbn1_walls", "hrsw_npt", "npt_periodic_log", "yukawa_npt",
+                                  "lj_exclusion_npt", "c1_periodic_nvt"])
 def test_windowed_sweeps_equal_all_partner_sweeps(oracle, name):
     """The reference's cost model (all N-1 partners per trial) and the window
     form make the same decisions and reach the same energies."""
