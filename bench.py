@@ -146,6 +146,11 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
     return out
 
 
+# autocorrelation lags kept per run for the common-window tau_int (windows
+# of up to 2000 sweeps, tau_int up to about 330 sweeps at c = 6)
+ACF_LAGS = 2000
+
+
 def accuracy_check(chains, opts, burnin, sweeps, bs, replicas, dump=""):
     """FP32 against FP64 per chain and observable at stated accuracy.
 
@@ -154,7 +159,9 @@ def accuracy_check(chains, opts, burnin, sweeps, bs, replicas, dump=""):
     `sweeps` sweeps with independent seeds: `replicas` FP64 reference runs,
     one more FP64 run (the null) and one FP32 run.  Per run and chain, the
     values of <U>, <L>, <rho>, cP, betaT and tau_int of U (stats.run_values;
-    tau_int compared through its logarithm).
+    tau_int at one window per chain for every run, the median of the
+    reference runs' own windows, stats.common_window_tau, and compared
+    through its logarithm).
     The null and the FP32 run are each compared with the reference runs
     (stats.replica_compare): t = (x - mean) / (sd sqrt(1 + 1/K)) per chain,
     Student's t with K - 1 degrees of freedom if the run is one more draw
@@ -198,16 +205,22 @@ def accuracy_check(chains, opts, burnin, sweeps, bs, replicas, dump=""):
             e.set_states(x, L0)
             e.run(sweeps)
             if pending is not None:  # the previous run's values while this one runs
-                vals[pending[0]] = stats.run_values(pending[1], pending[2], N, P, T, workers=workers)
+                vals[pending[0]] = stats.run_values(pending[1], pending[2], N, P, T, workers=workers,
+                                                    acf_lags=ACF_LAGS)
             smp = e.read_samples(sweeps)
         pending = (name, np.ascontiguousarray(smp["U"]), np.ascontiguousarray(smp["L"]))
         del smp
-    vals[pending[0]] = stats.run_values(pending[1], pending[2], N, P, T, workers=workers)
+    vals[pending[0]] = stats.run_values(pending[1], pending[2], N, P, T, workers=workers, acf_lags=ACF_LAGS)
     del pending
-    if dump:
-        np.savez_compressed(dump, N=N, P=P, T=T, potential=pot,
-                            **{f"{r}/{k}": v for r, d in vals.items() for k, v in d.items()})
     ref = [vals[f"ref{j}"] for j in range(replicas)]
+    own_tau = {r: d["tau_U"] for r, d in vals.items()}
+    windows = stats.common_window_tau(ref, list(vals.values()))
+    for d in vals.values():
+        del d["_tau_cum"], d["_tau_window"]
+    if dump:
+        np.savez_compressed(dump, N=N, P=P, T=T, potential=pot, tau_window=windows,
+                            **{f"{r}/{k}": v for r, d in vals.items() for k, v in d.items()},
+                            **{f"{r}/tau_U_own_window": v for r, v in own_tau.items()})
     null = stats.replica_compare(ref, vals["null"], groups=groups)
     fp32 = stats.replica_compare(ref, vals["fp32"], groups=groups)
     # observables constant in the ensemble (the length at constant length)
@@ -222,7 +235,12 @@ def accuracy_check(chains, opts, burnin, sweeps, bs, replicas, dump=""):
     return {"method": f"common start after an FP64 burn-in; {replicas} FP64 reference runs, one FP64 null run "
                       f"and one FP32 run with independent seeds; per chain t = (x - mean_ref) / "
                       f"(sd_ref sqrt(1 + 1/{replicas})), Student's t with {replicas - 1} degrees of freedom "
-                      f"under the hypothesis, band = its two-sided 99.73 % quantile (the probability of 3 sigma)",
+                      f"under the hypothesis, band = its two-sided 99.73 % quantile (the probability of 3 sigma); "
+                      f"tau_U of every run at one window per chain, the median of the reference runs' "
+                      f"Sokal windows (c = 6), compared through its logarithm",
+            "tau_window_sweeps": {"median": float(np.median(windows)), "max": int(windows.max()),
+                                  "chains_beyond_kept_lags": int((windows > ACF_LAGS).sum()),
+                                  "kept_lags": ACF_LAGS},
             "burnin_sweeps": burnin, "sweeps_per_run": sweeps, "reference_runs": replicas,
             "chains": len(chains), "band": null["U"]["band"], "gate_observables": gated,
             "gate_valid_null_ge_0.99": bool(valid), "gate_passed": bool(passed), "observables": per_obs,

@@ -309,14 +309,20 @@ def fluctuations_from_samples(U, L, P: float, T: float, nblocks: int = 20):
     return out
 
 
-def _tau_rows_fast(X, c: float = 6.0, chunk: int = 512, workers: int = 8):
-    """_tau_rows's tau for many rows, FFTs on `workers` threads."""
+def _tau_rows_fast(X, c: float = 6.0, chunk: int = 512, workers: int = 8, lags: int = 0):
+    """_tau_rows's tau for many rows, FFTs on `workers` threads.  lags > 0:
+    also each row's window (the column k of its running sum, lag k + 1) and
+    the running sums 0.5 + sum_{l <= j + 1} rho(l) for the first `lags`
+    columns (float32), from which tau_at_windows reads tau at any window."""
     import scipy.fft as sfft
     X = np.asarray(X, dtype=np.float64)
     nc, n = X.shape
     out = np.empty(nc)
     m = sfft.next_fast_len(2 * n, real=True)
     W = np.arange(1, n)
+    ncol = max(1, min(lags, n - 1))
+    wins = np.empty(nc, dtype=np.int64)
+    cums = np.empty((nc, ncol), dtype=np.float32) if lags else None
     for lo in range(0, nc, chunk):
         Y = X[lo:lo + chunk]
         Y = Y - Y.mean(axis=1, keepdims=True)
@@ -328,16 +334,44 @@ def _tau_rows_fast(X, c: float = 6.0, chunk: int = 512, workers: int = 8):
         ok = W[None, :] >= c * cum
         k = np.where(ok.any(axis=1), ok.argmax(axis=1), n - 2)
         out[lo:lo + chunk] = np.maximum(cum[np.arange(Y.shape[0]), k], 0.5)
-    return out
+        wins[lo:lo + chunk] = k
+        if lags:
+            cums[lo:lo + chunk] = cum[:, :ncol]
+    return (out, wins, cums) if lags else out
 
 
-def run_values(U, L, N, P, T, nblocks: int = 20, workers: int = 8):
+def tau_at_windows(cums, windows):
+    """tau_int of every row at its given window column (clamped to the
+    stored columns), at least 1/2: the running sum _tau_rows_fast keeps."""
+    cums = np.asarray(cums)
+    k = np.minimum(np.asarray(windows, dtype=np.int64), cums.shape[1] - 1)
+    return np.maximum(cums[np.arange(cums.shape[0]), k].astype(np.float64), 0.5)
+
+
+def common_window_tau(ref_runs, runs):
+    """tau_int of U at one window per chain for every run: the median over
+    the reference runs of their own Sokal windows (rounded down).  A run's
+    own window moves with its noise (a slow fluctuation lengthens it and
+    the estimate with it, which gives the per-run estimates a heavy upper
+    tail); at a common window every run's estimate is the same sum of
+    autocorrelations, and the window does not depend on the runs compared.
+    ref_runs, runs: run_values() results made with acf_lags > 0; each run's
+    "tau_U" is replaced.  Returns the windows (as lags, column + 1)."""
+    Kw = np.stack([np.asarray(r["_tau_window"]) for r in ref_runs])
+    w = np.floor(np.median(Kw, axis=0)).astype(np.int64)
+    for r in runs:
+        r["tau_U"] = tau_at_windows(r["_tau_cum"], w)
+    return w + 1
+
+
+def run_values(U, L, N, P, T, nblocks: int = 20, workers: int = 8, acf_lags: int = 0):
     """One run's per-chain values of every observable in OBSERVABLES, from
     per-sweep samples U, L [chain, sweep]: <U>, <L>, <rho> = <N/L>; cP and
     betaT as chain_observables defines them (the formulas of
     scripts/Analyze_Mean.py:164-176 per block of n/nblocks sweeps, averaged
     over the blocks); tau_int of U (Sokal's window, c = 6).  Returns
-    {observable: array[chain]}."""
+    {observable: array[chain]}; with acf_lags > 0 also "_tau_window" and
+    "_tau_cum" (the window and running sums, for common_window_tau)."""
     U = np.asarray(U, dtype=np.float64)
     L = np.asarray(L, dtype=np.float64)
     nc, n = U.shape
@@ -355,7 +389,10 @@ def run_values(U, L, N, P, T, nblocks: int = 20, workers: int = 8):
     else:
         out["cP"] = np.full(nc, np.nan)
         out["betaT"] = np.full(nc, np.nan)
-    out["tau_U"] = _tau_rows_fast(U, workers=workers)
+    if acf_lags:
+        out["tau_U"], out["_tau_window"], out["_tau_cum"] = _tau_rows_fast(U, workers=workers, lags=acf_lags)
+    else:
+        out["tau_U"] = _tau_rows_fast(U, workers=workers)
     return out
 
 

@@ -317,3 +317,45 @@ def test_band_allowance():
     p_le = lambda k, n: sum(math.comb(n, j) * q ** j * (1 - q) ** (n - j) for j in range(k + 1))
     assert p_le(1, 64) < 0.99 <= p_le(2, 64)
     assert stats.band_allowance(64) == 2
+
+
+def _ar1_rows(a, nc, n, rng):
+    """nc independent stationary AR(1) rows (x_t = a x_{t-1} + e_t)."""
+    from scipy.signal import lfilter
+    e = rng.standard_normal((nc, n))
+    e[:, 0] /= math.sqrt(1 - a * a)
+    return lfilter([1.0], [1.0, -a], e, axis=1)
+
+
+def test_common_window_tau():
+    """tau_int at one window per chain: the window is the median of the
+    reference runs' own Sokal windows; every run's value is its running
+    autocorrelation sum there, which equals its own estimate where its own
+    window is that window; on AR(1) rows (a = 0.8) the estimates centre on
+    (1 + a) / (2 (1 - a)) = 4.5; a null run drawn like the references lies
+    within the band, a run with a = 0.9 (tau_int 9.5) does not."""
+    rng = np.random.default_rng(26)
+    K, nc, n, a = 16, 192, 4000, 0.8
+    N, P, T = np.full(nc, 100), np.full(nc, 0.5), np.full(nc, 1.0)
+    L = np.full((nc, n), 200.0) + 0.01 * rng.standard_normal((nc, n))
+
+    def one(aa):
+        return stats.run_values(_ar1_rows(aa, nc, n, rng), L, N, P, T, workers=2, acf_lags=300)
+
+    ref = [one(a) for _ in range(K)]
+    null, slow = one(a), one(0.9)
+    own = {id(r): r["tau_U"].copy() for r in ref + [null]}
+    w = stats.common_window_tau(ref, ref + [null, slow])
+    med = np.floor(np.median(np.stack([r["_tau_window"] for r in ref]), axis=0)).astype(int)
+    assert np.array_equal(w, med + 1)
+    for r in ref + [null]:
+        same = r["_tau_window"] == med
+        assert same.any()
+        assert np.allclose(r["tau_U"][same], own[id(r)][same], rtol=1e-6)
+        k = np.minimum(med, r["_tau_cum"].shape[1] - 1)
+        assert np.array_equal(r["tau_U"], np.maximum(r["_tau_cum"][np.arange(nc), k].astype(np.float64), 0.5))
+    assert np.median(np.concatenate([r["tau_U"] for r in ref])) == pytest.approx((1 + a) / (2 * (1 - a)), rel=0.05)
+    ok = stats.replica_compare(ref, null)["tau_U"]
+    assert ok["outside"] <= ok["allowed_outside"], ok
+    bad = stats.replica_compare(ref, slow)["tau_U"]
+    assert bad["frac_within_band"] < 0.5, bad
