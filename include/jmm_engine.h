@@ -171,8 +171,11 @@ const char *jmm_last_error(void);
 
 /* The trial schedule of one chain: B blocks of particles, M_min = smallest
  * index separation of trials that run at once, w = largest index distance of
- * an interacting partner.  Pure host function; no reference counterpart (the
- * reference draws trial types with replacement, src/jmmMCState.cpp:1762). */
+ * an interacting partner (floor(rc / g_min) with a 1e-3 guard, g_min the
+ * hard core, which every state keeps between neighbours; M_min = w + 1,
+ * at least 2; NBN caps both).  Pure host function; no reference
+ * counterpart (the reference draws trial types with replacement,
+ * src/jmmMCState.cpp:1762). */
 int jmm_schedule(const jmm_chain_spec *spec, uint64_t *B, uint64_t *M_min, uint64_t *w);
 
 /* Replaces setupMCS (src/jmmMCState.cpp:261-790: parameter copy :267-290,
@@ -262,7 +265,7 @@ int jmm_set_states(jmm_engine *e, const double *x, const double *L, const double
  * maxDisAdjust/maxDVAdjust, src/jmmMCState.cpp:2100-2139, rescale maxStep and
  * maxdl in place): max_step[c] and max_dv[c] per chain, NULL leaves that kind
  * unchanged.  max_step may not exceed the chain's value at creation (the
- * block schedule and the periodic length bound are built for it).
+ * periodic length bound, L > 2 (rc + max_step), is built for it).
  * Synchronous. */
 int jmm_set_steps(jmm_engine *e, const double *max_step, const double *max_dv);
 

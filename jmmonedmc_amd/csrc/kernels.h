@@ -11,13 +11,16 @@
 //   slots    DevSlot[...]      chain -> lane range and LDS offset in a bundle
 //
 // Sweep schedule (shared with the oracle): chain c's N particles form B_c
-// blocks (B_c a power of two, every block at least M_min particles); in phase
-// t every block updates its t-th particle.  Trials in one phase are at least
-// M_min apart in index order, and M_min is chosen so that no two of them can
-// interact or read each other's particle (hard core g_min, cutoff rc, step
-// delta: M_min = floor((rc + 2 delta)/g_min) + 1, or NBN + 1, and at least 2
-// because the hard-core test reads both nearest partners).  A phase is
-// therefore equivalent to running its trials one after another in any order.
+// blocks (B_c a power of two or whole waves, every block at least M_min
+// particles); in phase t every block updates its t-th particle.  Trials in
+// one phase are at least M_min apart in index order, and M_min is chosen so
+// that no two of them can interact or read each other's particle: every
+// neighbour gap stays at least the hard core g_min (moves that would break
+// it are rejected), so a partner within the cutoff rc is at most
+// w = floor(rc/g_min) places away, and M_min = w + 1 (or NBN + 1, and at
+// least 2 because the hard-core test reads both nearest partners).  A phase
+// is therefore equivalent to running its trials one after another in any
+// order.
 //
 // LDS layout of a chain inside a bundle (batched forms): ghost-left (wg),
 // B blocks at an odd stride Sp, ghost-right (wg); see LdsChain.

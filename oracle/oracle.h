@@ -64,8 +64,13 @@
  *                      the closed forms 48/d^12 - 24/d^6, 576/d^12 - 144/d^6
  *   orc_sweeps_virial  the same samples and state as orc_sweeps, and after
  *                      every sweep the virials orc_virials_x gives
- *   schedule           the LJ exclusion distance (pot_p[2]) gives the
- *                      engine's M_min and w (C2 at 0.5 and 0.75 sigma)
+ *   schedule           the engine's (B, M_min, w) for LJ at two exclusion
+ *                      distances, the hard rod + square well and Yukawa,
+ *                      with rc / g_min on both sides of an integer; the
+ *                      window holds every interacting partner: single
+ *                      trials with steps up to four times max_step against
+ *                      brute-force energies, windowed against all-partner
+ *                      sweeps
  */
 #ifndef JMM_ORACLE_H
 #define JMM_ORACLE_H
