@@ -101,6 +101,28 @@ def test_fp32_kernel_within_tolerance(gpu, oracle, case):
     assert np.abs(grad - rg).max() <= 10 * TOL_FP32 * np.abs(rg).max()
 
 
+def test_configurations_inside_the_hard_core(gpu, oracle):
+    """Configurations handed to the backward need not keep the hard core:
+    LJ rings with gaps down to 0.3 sigma (the exclusion is 0.5 sigma) put
+    up to 10 partners within the cutoff where the chain's window holds 6;
+    the FP64 kernel walks every partner to the cutoff and its U equals the
+    all-pairs energy (the windowed sum does not)."""
+    chain = c4_chain()
+    N, g0 = 128, 0.3
+    L0 = N * 0.36
+    print(f"[gpu] backward below the hard core N={N}", flush=True)
+    X = ring_configs(4, N, L0, g0, seed=9)
+    L = np.full(4, L0)
+    obs, _ = _device_obs(chain, X, L, 64)
+    missed = 0
+    for c in range(4):
+        s = oracle_spec(oracle, dataclasses.replace(chain, N=N, L0=L0))
+        ub = oracle.brute_energy_x(s, X[c], L0)
+        assert abs(obs[c, 0] - ub) <= 1e-11 * abs(ub), (c, obs[c, 0], ub)
+        missed += oracle.full_energy_x(s, X[c], L0) != ub
+    assert missed > 0
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("precision", [32, 64])
 def test_theta_gradients_match_finite_differences(gpu, case, precision):
