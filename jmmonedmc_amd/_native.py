@@ -22,7 +22,7 @@ EXPORTED = [
     "jmm_abi_version", "jmm_last_error", "jmm_schedule", "jmm_create", "jmm_destroy",
     "jmm_run_sweeps", "jmm_synchronize", "jmm_full_energy", "jmm_read_samples", "jmm_read_samples_virial", "jmm_read_samples_noalloc",
     "jmm_get_state", "jmm_get_states", "jmm_set_state", "jmm_set_states", "jmm_perturb_energy", "jmm_get_counters",
-    "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_pair_observables",
+    "jmm_read_accept_log", "jmm_kernel_time", "jmm_observables", "jmm_observables_device", "jmm_pair_observables",
     "jmm_hist_configure", "jmm_hist_sample", "jmm_hist_read", "jmm_set_steps",
     "jmm_scaled_energy", "jmm_set_lengths", "jmm_chain_layout",
 ]
@@ -115,6 +115,7 @@ def lib(path: str = LIB):
     L.jmm_read_accept_log.argtypes = [vp, C.POINTER(C.c_uint8), C.c_uint32, u32p]
     L.jmm_kernel_time.argtypes = [vp, u64p, dp, u64p, C.c_int]
     L.jmm_observables.argtypes = [vp, dp, dp]
+    L.jmm_observables_device.argtypes = [vp, vp, vp]
     L.jmm_pair_observables.argtypes = [C.POINTER(ChainSpec), C.c_int, vp, vp, vp, vp, C.c_int, vp]
     L.jmm_hist_configure.argtypes = [vp, C.POINTER(HistSpec), C.c_uint32]
     L.jmm_hist_read.argtypes = [vp, u64p, u64p, u64p, C.c_int]

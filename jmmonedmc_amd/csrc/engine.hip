@@ -1335,8 +1335,11 @@ int jmm_get_counters(jmm_engine *e, int chain, jmm_counters *out)
     return 0;
 }
 
+// obs/grad: host arrays (copied out after the kernel, synchronous), or with
+// on_device the caller's device arrays, written by the kernel on the
+// engine's stream and not waited for.
 static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32,
-                            const double *Lnew_dev = nullptr)
+                            const double *Lnew_dev = nullptr, bool on_device = false)
 {
     HIPCHK(hipSetDevice(e->device));
     if (int rc = ensure_ext(e)) return rc;
@@ -1350,7 +1353,7 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32,
         std::iota(all.begin(), all.end(), 0u);
         HIPCHK(hipMemcpy(e->d_all, all.data(), sizeof(uint32_t) * e->n, hipMemcpyHostToDevice));
     }
-    if (grad && !e->d_grad)
+    if (grad && !on_device && !e->d_grad)
         if (int rc = dalloc(&e->d_grad, e->total_N)) return rc;
     HIPCHK(launch_lengths(e->d_states, e->d_Ltmp, (uint32_t)e->n, e->stream));
     ObsArgs a{};
@@ -1359,14 +1362,15 @@ static int observables_impl(jmm_engine *e, double *obs, double *grad, int fp32,
     a.L = e->d_Ltmp;
     a.list = e->d_all;
     a.Lnew = Lnew_dev;
-    a.obs = e->d_obs;
-    a.grad = grad ? e->d_grad : nullptr;
+    a.obs = on_device ? obs : e->d_obs;
+    a.grad = on_device ? grad : (grad ? e->d_grad : nullptr);
     // results in the caller's chain order, copied out as they are
     a.row = e->d_ext;
     a.goff = e->d_extoff;
     a.n = (uint32_t)e->n;
     a.standalone = 0;
     HIPCHK(launch_observables(a, fp32, (uint32_t)e->n, e->stream));
+    if (on_device) return 0;
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(obs, e->d_obs, sizeof(double) * (size_t)e->n * 6, hipMemcpyDeviceToHost));
     if (grad) HIPCHK(hipMemcpy(grad, e->d_grad, sizeof(double) * e->total_N, hipMemcpyDeviceToHost));
@@ -1458,6 +1462,13 @@ int jmm_observables(jmm_engine *e, double *obs, double *grad)
     if (!e || !obs) return fail(JMM_EINVAL, "jmm_observables: null argument");
     HIPCHK(hipSetDevice(e->device));
     return observables_impl(e, obs, grad, e->form == JMM_FORM_BATCHED_FP32);
+}
+
+int jmm_observables_device(jmm_engine *e, double *obs_dev, double *grad_dev)
+{
+    if (!e || !obs_dev) return fail(JMM_EINVAL, "jmm_observables_device: null argument");
+    HIPCHK(hipSetDevice(e->device));
+    return observables_impl(e, obs_dev, grad_dev, e->form == JMM_FORM_BATCHED_FP32, nullptr, true);
 }
 
 int jmm_full_energy(jmm_engine *e, double *U_out)

@@ -246,6 +246,14 @@ class Engine:
             return obs
         return obs, [g[o: o + n] for o, n in zip(self.offsets, self.Ns)]
 
+    def observables_device(self, obs_ptr: int, grad_ptr: int | None = None):
+        """observables() into device arrays (obs [n*6], grad [sum N]),
+        queued on the engine's stream after the sweeps before it; sync()
+        waits for it."""
+        check(lib().jmm_observables_device(self.h, C.c_void_p(obs_ptr),
+                                           C.c_void_p(grad_ptr) if grad_ptr else None),
+              "jmm_observables_device")
+
     def hist_configure(self, every: int, rhonb: int = 0, rbw: float = 0.0, gns: int = 0,
                        gsw: float = 0.0, gnb: int = 0, gbw: float = 0.0):
         """Sample rho(x) and g(x) every `every` sweeps (0 = off)."""

@@ -112,6 +112,7 @@ static void errors_mode()
     EXPECT(jmm_run_sweeps(nullptr, 1) != 0, "run null");
     EXPECT(jmm_synchronize(nullptr) != 0, "sync null");
     EXPECT(jmm_full_energy(nullptr, &d) != 0, "energy null");
+    EXPECT(jmm_observables_device(nullptr, nullptr, nullptr) != 0, "device observables null");
     EXPECT(jmm_read_samples(nullptr, &smp, 1, &n) != 0, "samples null");
     EXPECT(jmm_get_state(nullptr, 0, nullptr, &d, &d, nullptr) != 0, "state null");
     EXPECT(jmm_get_states(nullptr, nullptr, &d, &d) != 0, "states null");
@@ -191,6 +192,25 @@ static void exercise(int form, uint32_t seg_lds)
            "full energy %g vs %g", Uf[0], cnt[0].U);
     std::vector<double> obs(n * 6), grad(total);
     EXPECT(jmm_observables(e, obs.data(), grad.data()) == 0, "observables");
+    {
+        // the same into device arrays: equal bits
+        double *od = nullptr, *gd = nullptr;
+        if (hipMalloc((void **)&od, sizeof(double) * n * 6) == hipSuccess &&
+            hipMalloc((void **)&gd, sizeof(double) * total) == hipSuccess) {
+            EXPECT(jmm_observables_device(e, od, gd) == 0 && jmm_synchronize(e) == 0, "observables on device");
+            std::vector<double> obs2(n * 6), grad2(total);
+            EXPECT(hipMemcpy(obs2.data(), od, sizeof(double) * n * 6, hipMemcpyDeviceToHost) == hipSuccess &&
+                       hipMemcpy(grad2.data(), gd, sizeof(double) * total, hipMemcpyDeviceToHost) == hipSuccess,
+                   "observables on device: copy");
+            EXPECT(std::memcmp(obs2.data(), obs.data(), sizeof(double) * n * 6) == 0 &&
+                       std::memcmp(grad2.data(), grad.data(), sizeof(double) * total) == 0,
+                   "observables on device equal the host copy");
+        } else {
+            EXPECT(false, "hipMalloc for the device observables");
+        }
+        if (od) (void)hipFree(od);
+        if (gd) (void)hipFree(gd);
+    }
     jmm_hist_spec hs;
     std::memset(&hs, 0, sizeof hs);
     hs.rhonb = 16;

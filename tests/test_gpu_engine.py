@@ -257,3 +257,23 @@ def test_autotuner_candidates_give_identical_chains(gpu, tmp_path):
     assert r1.best in cands and len(r1.ms_per_sweep) == len(cands)
     r2 = tune(chains, block_sweeps=20, blocks=1, cache=cache)
     assert r2.best == r1.best and r2.ms_per_sweep == r1.ms_per_sweep
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["batched_fp64", "batched_fp32"])
+def test_observables_on_device_equal_host(gpu, form):
+    """jmm_observables_device, queued after a sweep block on the engine's
+    stream, writes the bits jmm_observables copies to the host."""
+    import torch
+    chains = CASES["mixed_batch"]()
+    print(f"[gpu] observables on device {form}", flush=True)
+    with Engine(chains, form=form, sample_capacity=10, block_sweeps=10) as e:
+        obs_d = torch.full((len(chains) * 6,), float("nan"), dtype=torch.float64, device="cuda")
+        grad_d = torch.full((int(sum(c.N for c in chains)),), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        e.run(10)
+        e.observables_device(obs_d.data_ptr(), grad_d.data_ptr())
+        e.sync()
+        obs, grads = e.observables(want_grad=True)
+    assert np.array_equal(obs_d.cpu().numpy().reshape(-1, 6), obs)
+    assert np.array_equal(grad_d.cpu().numpy(), np.concatenate(grads))
