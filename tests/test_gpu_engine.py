@@ -267,10 +267,11 @@ def test_observables_on_device_equal_host(gpu, form):
     import torch
     chains = CASES["mixed_batch"]()
     print(f"[gpu] observables on device {form}", flush=True)
+    # torch's device first (its own HIP runtime), then the engine's
+    obs_d = torch.full((len(chains) * 6,), float("nan"), dtype=torch.float64, device="cuda")
+    grad_d = torch.full((int(sum(c.N for c in chains)),), float("nan"), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
     with Engine(chains, form=form, sample_capacity=10, block_sweeps=10) as e:
-        obs_d = torch.full((len(chains) * 6,), float("nan"), dtype=torch.float64, device="cuda")
-        grad_d = torch.full((int(sum(c.N for c in chains)),), float("nan"), dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
         e.run(10)
         e.observables_device(obs_d.data_ptr(), grad_d.data_ptr())
         e.sync()
