@@ -191,9 +191,11 @@ def test_engine_observables_and_reweighting(gpu, oracle):
     jmm_observables equals the oracle on the same states, and the
     fluctuation gradient of <W> equals the derivative of the reweighted
     average on the same samples (an identity at dtheta -> 0)."""
+    import torch
     from jmmonedmc_amd.autograd import fluctuation_gradient
     from jmmonedmc_amd.engine import Chain, Engine
     from helpers import lj
+    torch.cuda.init()  # torch's device first (its own HIP runtime), then the engine's
     chains = [Chain(N=256, rho=0.6, T=1.0, seed=3, chain_id=c, **lj()) for c in range(16)]
     snaps_x, snaps_L = [], []
     with Engine(chains, form="batched_fp32", block_sweeps=50) as eng:
