@@ -157,15 +157,20 @@ def test_c3_random_chains_fp32_ties(gpu, oracle):
 
 
 @pytest.mark.gpu
-def test_c2x_chain_fp32_full_length_ties(gpu, oracle):
-    """One chain of C2 with the LJ exclusion distance at 0.75 sigma (4096
-    particles, 512 blocks, the 512-lane kernel), FP32 form at the full 1e4
-    sweeps: every flip a tie within its bound, state bit for bit equal to
-    the oracle's after every chunk of 2500 sweeps."""
-    ch = configs.c2(n_chains=1, exclusion=0.75, bcap=1024)[0]
-    print("[gpu] C2x chain FP32 ties", flush=True)
+@pytest.mark.parametrize("cfg", ["c2", "c2x", "c4"])
+def test_config_chain_fp32_full_length_ties(gpu, oracle, cfg):
+    """One chain of C2 (4096 particles, NPT, 512 blocks, the 512-lane
+    kernel), of C2 with the LJ exclusion distance at 0.75 sigma, or of C4
+    (1024 particles, NVT at rho = 0.6, 128 blocks), FP32 form at the full
+    1e4 sweeps: every flip a tie within its bound, state bit for bit equal
+    to the oracle's after every chunk of 2500 sweeps."""
+    if cfg == "c2x":
+        ch = configs.c2(n_chains=1, exclusion=0.75)[0]
+    else:
+        ch = getattr(configs, cfg)(n_chains=1)[0]
+    print(f"[gpu] {cfg} chain FP32 ties", flush=True)
     got = run_fp32_follow(oracle, [ch], 10_000, 2_500, 500)
-    n = check_replay(got, "c2x chain", 10_000)
+    n = check_replay(got, f"{cfg} chain", 10_000)
     assert n <= 1e-5 * 10_000 * ch.N
 
 
