@@ -1530,9 +1530,11 @@ __device__ __forceinline__ void run_segment(const SegArgs &g, const DevSeg &sg, 
                 const double xn = xi + md;
                 ++n_tr;
                 double dE = 0.0;
-                const bool accd = trial_lds<Real, KIND, LPB == 2>(c, q, gmin, beta, betaf, i, t, size, my,
-                                                                  gapL, gapR, L, xi, xn, md, u, dE,
-                                                                  n_terms, n_wall, n_core, side);
+                // one lane per block, FP32: two partners per side per walk
+                // step, as the whole-chain kernel (the same sums, bit for bit)
+                const bool accd = trial_lds<Real, KIND, LPB == 2, LPB == 1 && sizeof(Real) == 4>(
+                    c, q, gmin, beta, betaf, i, t, size, my, gapL, gapR, L, xi, xn, md, u, dE, n_terms, n_wall,
+                    n_core, side);
                 if (own) {
                     const double xs = accd ? xn : xi;
                     c.R[my] = xs;
