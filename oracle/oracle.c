@@ -319,8 +319,8 @@ struct orc_chain {
     orc_spec s;
     double *x;
     double L, U, beta, shift, gmin;
-    double step_cap; /* max_step at creation: the schedule and the periodic
-                        length bound are built for it */
+    double step_cap; /* max_step at creation: the periodic length bound,
+                        L > 2 (rc + max_step), is built for it */
     uint64_t sweep, B, M_min, w;
     uint64_t cnt[8]; /* trials, acc_d, wall_rej, hc_rej, pair_terms, vol_trials, acc_v, recomputes */
     double max_drift;
