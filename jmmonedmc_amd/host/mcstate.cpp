@@ -129,8 +129,9 @@ static jmm_chain_spec spec_from_input(const MCInput &inp, unsigned long k, bool 
     // NPT starts at l = N (src/jmmMCState.cpp:549-555) unless LSTART is given
     s.L0 = npt ? (inp.lStart > 0 ? inp.lStart : (double)inp.N) : inp.L;
     // with DADJ adaptation the step may grow to the reference's clamp 0.5
-    // (src/jmmMCState.cpp:2106-2111): the schedule is built for the larger
-    // of the two and the INPUT value is set right after creation
+    // (src/jmmMCState.cpp:2106-2111): the engine's step cap (and periodic
+    // length bound) is built for the larger of the two and the INPUT value
+    // is set right after creation
     const bool adapt = inp.mdai && inp.mdai < inp.ns;
     s.max_step = adapt ? std::max(inp.maxStep, 0.5) : inp.maxStep;
     s.max_dv = inp.maxdl;
